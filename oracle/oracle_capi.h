@@ -1,0 +1,106 @@
+/* ORACLE — test infrastructure only.  C ABI of the CPU FP64 restatement of
+ * quasar's 2D CT ideal-MHD step, loaded by tests/ and bench.py's CPU-baseline
+ * leg through ctypes.  Not part of the product library. */
+#ifndef QUASAR_MHD_ORACLE_CAPI_H
+#define QUASAR_MHD_ORACLE_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int32_t nx, ny, ng, pad_;
+  double lx, ly, ox, oy, cell_dx, cell_dy;
+} OrcGrid;
+
+/* Same field order and meaning as the product's qmhd_ic_spec. */
+typedef struct {
+  int32_t kind, pad_;
+  double gamma, magnetic_scale, interface_x;
+  double left[8], right[8];
+  double rho, pressure, b0, total_b0, amplitude;
+  int64_t wavenumber;
+  double magnetic_velocity_scale;
+  double b_uniform[3];
+  double center[2];
+  double r_in, r0, r1, rho_in, rho_out, p_in, p_out, p_core, p_ambient, rho_ambient, u0;
+  double ot_rho, ot_p;
+  double noise_amplitude;
+  uint64_t noise_seed;
+} OrcIcSpec;
+
+typedef struct {
+  long long mp_clips, muscl_fallbacks, pc_fallbacks, sentinels, point_fallbacks, hll_fallbacks;
+} OrcCounters;
+
+/* Per-state numerics (arrays of 8 in the order rho,mx,my,mz,E,bx,by,bz). */
+double orc_cr_sum(const double* terms, int n);
+double orc_product_sum(const double* a, const double* b, int n);
+double orc_pressure(const double* u, double gamma);
+void orc_physical_flux_x(const double* u, double gamma, double* f);
+void orc_hll_flux_x(const double* l, const double* r, double sl, double sr, double gamma, double* f);
+void orc_lf_flux_x(const double* l, const double* r, double gamma, double* f);
+int orc_hlld_flux_x(const double* l, const double* r, double gamma, double* f);
+double orc_fast_speed(const double* u, int dir, double gamma);
+double orc_mp_limit(double vl, const double* v5);
+double orc_mp5_interp(const double* v5);
+double orc_mp7_interp(const double* v7);
+double orc_mp5_reconstruct(const double* v5);
+double orc_mp7_reconstruct(const double* v7);
+int orc_eigen_build(const double* u, double gamma, double* L49, double* R49, double* speeds7);
+void orc_to_char(const double* d8, const double* L49, double* w7);
+double orc_point_value(int order, int node, const double* samples);
+double orc_gauss_average(int order, const double* samples);
+double orc_philox_uniform_pm1(uint64_t seed, uint32_t component, uint64_t cell);
+void orc_philox4x32_10(uint32_t* ctr4, const uint32_t* key2);
+
+/* Grid kernels; planes are full padded storage (pitch*height doubles). */
+double orc_cell_bx(const OrcGrid* g, const double* bx_face, int i, int j, int co);
+double orc_cell_by(const OrcGrid* g, const double* by_face, int i, int j, int co);
+void orc_reconstruct(const OrcGrid* g, int dir, int order, const int* flags, double gamma,
+                     double* const* u8, double* const* out16, OrcCounters* counters);
+void orc_face_flux(const OrcGrid* g, int dir, int order, int hll_only, const int* flags,
+                   double gamma, double* const* in16, double* const* flux8, OrcCounters* counters);
+void orc_flux_difference(const OrcGrid* g, int dir, double* const* flux8, double* const* dudt8);
+void orc_ct_emf(const OrcGrid* g, int order, int hll_only, const int* flags, double gamma,
+                double* const* u8, double* const* ifx16, double* const* ify16, double* cell_ez,
+                double* xface_ez, double* yface_ez, int* xnj, int* ynj, double* ez_edge);
+void orc_emf_curl_rate(const OrcGrid* g, const double* ez, double* const* dudt8);
+void orc_rk_stage(const OrcGrid* g, double a, double b, double c, double* const* un8,
+                  double* const* us8, double* const* dudt8, double* const* out8);
+void orc_fill_ghosts(const OrcGrid* g, const int* fluid_flags, const int* field_flags,
+                     double* const* u8);
+double orc_cfl_max_rate(const OrcGrid* g, int order, const int* flags, double gamma,
+                        double* const* ifx16, double* const* ify16);
+double orc_admissible_fraction(const OrcGrid* g, double* const* base8, double* const* cand8,
+                               double rho_floor, double q_floor, int co);
+double orc_divb_linf(const OrcGrid* g, double* const* u8);
+double orc_divb_relative_linf(const OrcGrid* g, double* const* u8, int solver_owned);
+void orc_conserved_totals(const OrcGrid* g, double* const* u8, double* mass, double* energy);
+int orc_seed_initial_state(const OrcGrid* g, const OrcIcSpec* spec, double* const* out8);
+
+/* Solver (restatement of MhdSolver2D).  Status: 0 ok, 1 invalid argument,
+ * 2 runtime error.  orc_last_error() describes the last failure. */
+void* orc_solver_create(const OrcGrid* g, double gamma, double cfl, int order,
+                        const int* fluid_flags, const int* field_flags);
+void orc_solver_destroy(void* h);
+int orc_solver_seed(void* h, int comp, const double* data);
+int orc_solver_get(void* h, int comp, double* out);
+int orc_solver_get_collocated(void* h, int axis, double* out);
+int orc_solver_cfl_limit(void* h, double* dt);
+int orc_solver_step_unchecked(void* h, double dt);
+int orc_solver_step(void* h, double dt);
+int orc_solver_divergence_b_max(void* h, double* out);
+int orc_solver_conserved_totals(void* h, double* mass, double* energy);
+int orc_solver_compute_residual(void* h, double* const* dudt8);
+int orc_solver_last_substeps(void* h);
+void orc_solver_counters(void* h, OrcCounters* out);
+const char* orc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1368 @@
+// ORACLE — test infrastructure only. Never linked into the product library.
+// Grid-level restatement of the reference kernels; see oracle_grid.hpp.
+
+#include "oracle_grid.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+
+namespace oracle {
+namespace {
+
+constexpr Real kInf = std::numeric_limits<Real>::infinity();
+
+// include/quasar/physics/mhd/mhd_staggering.hpp:42-50
+const Real kFaceToCell8[8] = {-191, 1879, -9531, 68323, 68323, -9531, 1879, -191};
+const Real kFaceToCell6[6] = {11, -93, 802, 802, -93, 11};
+const Real kFaceToCell4[4] = {-1, 13, 13, -1};
+const Real kAverage2[2] = {1, 1};
+
+// include/quasar/numerics/finite_volume_quadrature.hpp:12-47
+const Real kMp5Gauss[3] = {Real{5} / Real{18}, Real{4} / Real{9}, Real{5} / Real{18}};
+const Real kMp5Point[3][5] = {
+    {-0.0392523473402346598, 0.312987195324173573, 0.910833333333333384,
+     -0.216320528657506839, 0.0317523473402346601},
+    {0.00468749999999999983, -0.0604166666666666671, 1.11145833333333344,
+     -0.0604166666666666671, 0.00468749999999999983},
+    {0.0317523473402346601, -0.216320528657506839, 0.910833333333333384,
+     0.312987195324173573, -0.0392523473402346598}};
+const Real kMp7Gauss[4] = {0.173927422568726925, 0.326072577431273103,
+                           0.326072577431273103, 0.173927422568726925};
+const Real kMp7Point[4][7] = {
+    {0.00860109764472105458, -0.0796691475531970394, 0.415576982717296983,
+     0.849155569794037945, -0.245805403605616202, 0.0591434746546156687,
+     -0.0070025736518584264},
+    {0.00327670662777140203, -0.0266359260617751309, 0.0928282644451935562,
+     1.08046056237311472, -0.183384586025290314, 0.0375843394851378479,
+     -0.00412936084415200903},
+    {-0.00412936084415200903, 0.0375843394851378479, -0.183384586025290314,
+     1.08046056237311472, 0.0928282644451935562, -0.0266359260617751309,
+     0.00327670662777140203},
+    {-0.0070025736518584264, 0.0591434746546156687, -0.245805403605616202,
+     0.849155569794037945, 0.415576982717296983, -0.0796691475531970394,
+     0.00860109764472105458}};
+
+Real comp(const State& s, int c) {
+  switch (c) {
+    case 0: return s.rho;
+    case 1: return s.mx;
+    case 2: return s.my;
+    case 3: return s.mz;
+    case 4: return s.energy;
+    case 5: return s.bx;
+    case 6: return s.by;
+    default: return s.bz;
+  }
+}
+void set_comp(State& s, int c, Real v) {
+  switch (c) {
+    case 0: s.rho = v; break;
+    case 1: s.mx = v; break;
+    case 2: s.my = v; break;
+    case 3: s.mz = v; break;
+    case 4: s.energy = v; break;
+    case 5: s.bx = v; break;
+    case 6: s.by = v; break;
+    default: s.bz = v; break;
+  }
+}
+
+// mhd_staggering.hpp:72-79
+int collocation_width(int extent, int ng) {
+  const int requested = (ng >= 4) ? 8 : (ng >= 3) ? 6 : (ng >= 2) ? 4 : 2;
+  if (requested >= 8 && extent >= 8) return 8;
+  if (requested >= 6 && extent >= 6) return 6;
+  if (requested >= 4 && extent >= 4) return 4;
+  return extent >= 2 ? 2 : 1;
+}
+
+// mhd_staggering.hpp:81-96
+Real barycentric_weight(int width, int k) {
+  if (width == 8) {
+    static const int w[8] = {1, -7, 21, -35, 35, -21, 7, -1};
+    return static_cast<Real>(w[k]);
+  }
+  if (width == 6) {
+    static const int w[6] = {1, -5, 10, -10, 5, -1};
+    return static_cast<Real>(w[k]);
+  }
+  if (width == 4) {
+    static const int w[4] = {1, -3, 3, -1};
+    return static_cast<Real>(w[k]);
+  }
+  return k == 0 ? Real{1} : Real{-1};
+}
+
+Real axis_sample(const Grid& g, const Real* face, int axis, int q, int fixed) {
+  return axis == 0 ? face[g.index(q, fixed)] : face[g.index(fixed, q)];
+}
+
+// mhd_staggering.hpp:111-173 (Cartesian, unit radial weights).
+Real one_sided_face_integral(const Grid& g, const Real* face, int axis, int fixed,
+                             int start, int width, int cell) {
+  const Real xq[4] = {0.0694318442029737124, 0.3300094782075718676,
+                      0.6699905217924281324, 0.9305681557970262876};
+  const Real wq[4] = {0.1739274225687269287, 0.3260725774312730713,
+                      0.3260725774312730713, 0.1739274225687269287};
+  const int rel = cell - start;
+  Real a[32], b[32], d0[32], d1[32];
+  int n = 0;
+  for (int q = 0; q < 4; ++q) {
+    const Real x = static_cast<Real>(rel) + xq[q];
+    Real da[8], db[8], dd0[8], dd1[8];
+    for (int k = 0; k < width; ++k) {
+      da[k] = barycentric_weight(width, k);
+      db[k] = Real{1};
+      dd0[k] = x - static_cast<Real>(k);
+      dd1[k] = Real{1};
+    }
+    const Real denominator = product_quotient_sum(da, db, dd0, dd1, width);
+    for (int k = 0; k < width; ++k) {
+      a[n] = (wq[q] / Real{1}) * barycentric_weight(width, k);
+      b[n] = axis_sample(g, face, axis, start + k, fixed);
+      d0[n] = x - static_cast<Real>(k);
+      d1[n] = denominator;
+      ++n;
+    }
+  }
+  return product_quotient_sum(a, b, d0, d1, n);
+}
+
+// mhd_staggering.hpp:175-236
+Real face_samples_to_cell_average(const Grid& g, const Real* face, int axis, int cell,
+                                  int fixed) {
+  const int n = axis == 0 ? g.nx : g.ny;
+  const int extent = n + 2 * g.ng;
+  const int width = collocation_width(extent, g.ng);
+  const int lo = -g.ng;
+  const int hi = n + g.ng - 1;
+  if (width == 1) return axis_sample(g, face, axis, cell, fixed);
+  const int left = width / 2 - 1;
+  const int centered_start = cell - left;
+  if (centered_start >= lo && centered_start + width - 1 <= hi) {
+    Real s[8];
+    for (int k = 0; k < width; ++k) s[k] = axis_sample(g, face, axis, centered_start + k, fixed);
+    if (width == 8) return rational_stencil(kFaceToCell8, s, 8, Real{120960});
+    if (width == 6) return rational_stencil(kFaceToCell6, s, 6, Real{1440});
+    if (width == 4) return rational_stencil(kFaceToCell4, s, 4, Real{24});
+    return rational_stencil(kAverage2, s, 2, Real{2});
+  }
+  int start = centered_start;
+  if (start < lo) start = lo;
+  if (start + width - 1 > hi) start = hi - width + 1;
+  return one_sided_face_integral(g, face, axis, fixed, start, width, cell);
+}
+
+// mhd_staggering.hpp:243-256
+Real low_order_face_to_cell_average(const Grid& g, const Real* face, int axis, int cell,
+                                    int fixed) {
+  const int n = axis == 0 ? g.nx : g.ny;
+  const int hi = n + g.ng - 1;
+  const Real lo_face = axis_sample(g, face, axis, cell, fixed);
+  if (cell >= hi) return lo_face;
+  const Real s[2] = {lo_face, axis_sample(g, face, axis, cell + 1, fixed)};
+  return rational_stencil(kAverage2, s, 2, Real{2});
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction (src/backend/hip/mhd/mhd_reconstruct.hip).
+
+int slope_drop(int c, int n, int lo, int hi) {  // mhd_reconstruct.hip:129-137
+  if (lo != 0 && lo != 4 && c <= 0) return -1;
+  if (hi != 0 && hi != 4 && c >= n - 1) return +1;
+  return 0;
+}
+
+Real limited_slope(Real back, Real fwd, int drop) {  // :96-100
+  if (drop < 0) return fwd;
+  if (drop > 0) return back;
+  return minmod2(back, fwd);
+}
+
+Prim muscl_face(const Prim& wm, const Prim& wc, const Prim& wp, Real sign, int drop) {
+  const Real h = Real{0.5} * sign;  // :107-120
+  Prim f;
+  f.rho = wc.rho + h * limited_slope(wc.rho - wm.rho, wp.rho - wc.rho, drop);
+  f.vx = wc.vx + h * limited_slope(wc.vx - wm.vx, wp.vx - wc.vx, drop);
+  f.vy = wc.vy + h * limited_slope(wc.vy - wm.vy, wp.vy - wc.vy, drop);
+  f.vz = wc.vz + h * limited_slope(wc.vz - wm.vz, wp.vz - wc.vz, drop);
+  f.p = wc.p + h * limited_slope(wc.p - wm.p, wp.p - wc.p, drop);
+  f.bx = wc.bx + h * limited_slope(wc.bx - wm.bx, wp.bx - wc.bx, drop);
+  f.by = wc.by + h * limited_slope(wc.by - wm.by, wp.by - wc.by, drop);
+  f.bz = wc.bz + h * limited_slope(wc.bz - wm.bz, wp.bz - wc.bz, drop);
+  return f;
+}
+
+void set_normal_b(State& s, int dir, Real v) {  // :141-153
+  Real& bn = (dir == 0) ? s.bx : s.by;
+  s.energy = product_sum3(s.energy, Real{1}, v, Real{0.5} * v, bn, -Real{0.5} * bn);
+  bn = v;
+}
+
+bool reconstructed_state_admissible(const State& s, Real gamma) {  // :433-446
+  if (!(std::isfinite(s.rho) && s.rho > Real{0})) return false;
+  const Prim w = to_primitive(s, gamma);
+  if (!(std::isfinite(w.vx) && std::isfinite(w.vy) && std::isfinite(w.vz) &&
+        std::isfinite(w.p) && w.p > Real{0}))
+    return false;
+  return std::isfinite(s.mx) && std::isfinite(s.my) && std::isfinite(s.mz) &&
+         std::isfinite(s.energy) && std::isfinite(s.bx) && std::isfinite(s.by) &&
+         std::isfinite(s.bz);
+}
+
+void muscl_reconstruct_face(const Grid& g, int dir, int i, int j, Real gamma, int side_lo,
+                            int side_hi, const Field& u, State& ul, State& ur) {  // :375-414
+  const int di = (dir == 0) ? 1 : 0;
+  const int dj = (dir == 0) ? 0 : 1;
+  const State sm_lo = load_cell_state(g, u, i - 2 * di, j - 2 * dj, 0);
+  const State sm_c = load_cell_state(g, u, i - di, j - dj, 0);
+  const State sm_hi = load_cell_state(g, u, i, j, 0);
+  const State sp_hi = load_cell_state(g, u, i + di, j + dj, 0);
+  const Prim wm_lo = to_primitive(sm_lo, gamma);
+  const Prim wm_c = to_primitive(sm_c, gamma);
+  const Prim wm_hi = to_primitive(sm_hi, gamma);
+  const Prim wp_hi = to_primitive(sp_hi, gamma);
+  const int n_dir = (dir == 0) ? g.nx : g.ny;
+  const int minus_idx = (dir == 0) ? i - di : j - dj;
+  const int plus_idx = (dir == 0) ? i : j;
+  const int drop_m = slope_drop(minus_idx, n_dir, side_lo, side_hi);
+  const int drop_p = slope_drop(plus_idx, n_dir, side_lo, side_hi);
+  const Prim wl = muscl_face(wm_lo, wm_c, wm_hi, Real{+1}, drop_m);
+  const Prim wr = muscl_face(wm_c, wm_hi, wp_hi, Real{-1}, drop_p);
+  ul = to_conserved(wl, gamma);
+  ur = to_conserved(wr, gamma);
+}
+
+// ---------------------------------------------------------------------------
+// Face quadrature (src/backend/hip/mhd/mhd_face_quadrature.hpp).
+
+bool state_is_admissible(const State& s, Real gamma) {  // :62-71
+  if (!(std::isfinite(s.rho) && s.rho > Real{0} && std::isfinite(s.mx) &&
+        std::isfinite(s.my) && std::isfinite(s.mz) && std::isfinite(s.energy) &&
+        std::isfinite(s.bx) && std::isfinite(s.by) && std::isfinite(s.bz)))
+    return false;
+  const Real p = pressure(s, gamma);
+  return std::isfinite(p) && p > Real{0};
+}
+
+// :73-101 (Cartesian): value at transverse Gauss node `node` from the
+// 5/7 neighbouring face averages `samples` (offsets -half..half).
+Real point_value(int order, int node, const Real* samples) {
+  if (order >= 7) return weighted_value(kMp7Point[node], samples, 7);
+  return weighted_value(kMp5Point[node], samples, 5);
+}
+
+std::size_t transverse_index(const Grid& g, int dir, int i, int j, int offset) {
+  return dir == 0 ? g.index(i, j + offset) : g.index(i + offset, j);
+}
+
+State load_side(const Iface& v, std::size_t k, bool left) {
+  const int o = left ? 0 : 8;
+  State s;
+  for (int c = 0; c < 8; ++c) set_comp(s, c, v.c[o + c][k]);
+  return s;
+}
+
+State point_interface_state(const Grid& g, int dir, int i, int j, int order, int node,
+                            const Iface& v, bool left) {  // :117-143
+  const int half = order >= 7 ? 3 : 2;
+  const int o = left ? 0 : 8;
+  State s;
+  for (int c = 0; c < 8; ++c) {
+    Real samples[7];
+    for (int k = 0; k <= 2 * half; ++k)
+      samples[k] = v.c[o + c][transverse_index(g, dir, i, j, k - half)];
+    set_comp(s, c, point_value(order, node, samples));
+  }
+  return s;
+}
+
+Flux riemann_flux(const State& L, const State& R, int dir, int hll_only, Real gamma,
+                  Counters* counters) {  // :194-205
+  const State Lr = rotate(L, dir);
+  const State Rr = rotate(R, dir);
+  int path = -1;
+  const Flux f = hll_only != 0 ? lax_friedrichs_flux_x(Lr, Rr, gamma)
+                               : hlld_flux_x(Lr, Rr, gamma, &path);
+  if (counters && path == kHllFallback) {
+#pragma omp atomic
+      counters->hll_fallbacks++;
+      }
+  return rotate(f, dir);
+}
+
+Flux average_face_flux(const Grid& g, int dir, int i, int j, int order, int hll_only,
+                       Real gamma, const Iface& v, Counters* counters) {  // :221-272
+  const std::size_t k = g.index(i, j);
+  const State base_l = load_side(v, k, true);
+  const State base_r = load_side(v, k, false);
+  if (order != 5 && order != 7) return riemann_flux(base_l, base_r, dir, hll_only, gamma, counters);
+  Flux flux[4];
+  const int count = order == 5 ? 3 : 4;
+  for (int q = 0; q < count; ++q) {
+    const State l = point_interface_state(g, dir, i, j, order, q, v, true);
+    const State r = point_interface_state(g, dir, i, j, order, q, v, false);
+    if (!state_is_admissible(l, gamma) || !state_is_admissible(r, gamma)) {
+      if (counters) {
+#pragma omp atomic
+      counters->point_fallbacks++;
+      }
+      return riemann_flux(base_l, base_r, dir, hll_only, gamma, counters);
+    }
+    flux[q] = riemann_flux(l, r, dir, hll_only, gamma, counters);
+  }
+  const Real* w = order == 5 ? kMp5Gauss : kMp7Gauss;
+  Flux out;
+  for (int c = 0; c < 8; ++c) {
+    Real samples[4];
+    for (int q = 0; q < count; ++q) samples[q] = comp(flux[q], c);
+    set_comp(out, c, weighted_value(w, samples, count));
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// CT helpers (src/backend/hip/mhd/mhd_ct.hip).
+
+Real scaled_face_average(Real e0, Real e1) {  // :46-55
+  if (e0 == e1) return e0;
+  if (!(std::isfinite(e0) && std::isfinite(e1))) return Real{0.5} * e0 + Real{0.5} * e1;
+  const Real s[2] = {e0, e1};
+  return rational_stencil(kAverage2, s, 2, Real{2});
+}
+
+Real edge5(const Real* e) {  // :57-73
+  if (e[0] == e[1] && e[0] == e[2] && e[0] == e[3] && e[0] == e[4] && e[0] == e[5]) return e[0];
+  bool finite = true;
+  for (int k = 0; k < 6; ++k) finite = finite && std::isfinite(e[k]);
+  if (!finite)
+    return (e[0] - Real{8} * e[1] + Real{37} * e[2] + Real{37} * e[3] - Real{8} * e[4] + e[5]) /
+           Real{60};
+  static const Real w[6] = {1, -8, 37, 37, -8, 1};
+  return rational_stencil(w, e, 6, Real{60});
+}
+
+Real edge7(const Real* e) {  // :75-95
+  bool constant = true;
+  for (int k = 1; k < 8; ++k) constant = constant && e[0] == e[k];
+  if (constant) return e[0];
+  bool finite = true;
+  for (int k = 0; k < 8; ++k) finite = finite && std::isfinite(e[k]);
+  if (!finite)
+    return (-Real{3} * e[0] + Real{29} * e[1] - Real{139} * e[2] + Real{533} * e[3] +
+            Real{533} * e[4] - Real{139} * e[5] + Real{29} * e[6] - Real{3} * e[7]) /
+           Real{840};
+  static const Real w[8] = {-3, 29, -139, 533, 533, -139, 29, -3};
+  return rational_stencil(w, e, 8, Real{840});
+}
+
+int face_index(int q, int n, int lo_mode, int hi_mode) {  // :146-158
+  if (lo_mode == 0 && hi_mode == 0) {
+    int w = q % n;
+    if (w < 0) w += n;
+    return w;
+  }
+  if (q < 0) return lo_mode == 4 ? q : 0;
+  if (q > n) return hi_mode == 4 ? q : n;
+  return q;
+}
+
+bool interface_is_equal_state(std::size_t k, const Iface& v) {  // :160-166
+  for (int c = 0; c < 8; ++c)
+    if (!(v.c[c][k] == v.c[8 + c][k])) return false;
+  return true;
+}
+
+bool face_problem_has_no_jump(const Grid& g, int dir, int i, int j, int order,
+                              const Iface& v, const Flags& m) {  // :175-198
+  const int recovery_half = order >= 7 ? 3 : order >= 5 ? 2 : 0;
+  for (int offset = -recovery_half; offset <= recovery_half; ++offset) {
+    const int si = dir == 0 ? i : i + offset;
+    const int sj = dir == 0 ? j + offset : j;
+    const int mi = face_index(si, g.nx, m.side[0], m.side[1]);
+    const int mj = face_index(sj, g.ny, m.side[2], m.side[3]);
+    if (mi < -g.ng || mi >= g.nx + g.ng || mj < -g.ng || mj >= g.ny + g.ng ||
+        !interface_is_equal_state(g.index(mi, mj), v))
+      return false;
+  }
+  return true;
+}
+
+bool directional_corner_stencil_has_no_jump(const Grid& g, int dir, int i, int j,
+                                            int order, const int* no_jump,
+                                            const Flags& m) {  // :205-239
+  const int lo = order >= 7 ? -4 : order >= 5 ? -3 : -1;
+  const int hi = order >= 7 ? 3 : order >= 5 ? 2 : 0;
+  for (int offset = lo; offset <= hi; ++offset) {
+    const int si = dir == 0 ? i : i + offset;
+    const int sj = dir == 0 ? j + offset : j;
+    const int ii = face_index(si, g.nx, m.side[0], m.side[1]);
+    const int jj = face_index(sj, g.ny, m.side[2], m.side[3]);
+    if (ii < -g.ng || ii >= g.nx + g.ng || jj < -g.ng || jj >= g.ny + g.ng ||
+        no_jump[g.index(ii, jj)] == 0)
+      return false;
+  }
+  return true;
+}
+
+Real table_at(const Grid& g, int i, int j, const Real* t, const Flags& m) {  // :246-260
+  const int ii = face_index(i, g.nx, m.side[0], m.side[1]);
+  const int jj = face_index(j, g.ny, m.side[2], m.side[3]);
+  return t[g.index(ii, jj)];
+}
+
+Real interp_x_to_corner(const Grid& g, int i, int j, int order, const Real* xt,
+                        const Flags& m) {  // :262-288
+  if (order >= 7) {
+    Real e[8];
+    for (int q = 0; q < 8; ++q) e[q] = table_at(g, i, j - 4 + q, xt, m);
+    return edge7(e);
+  }
+  if (order >= 5) {
+    Real e[6];
+    for (int q = 0; q < 6; ++q) e[q] = table_at(g, i, j - 3 + q, xt, m);
+    return edge5(e);
+  }
+  return scaled_face_average(table_at(g, i, j - 1, xt, m), table_at(g, i, j, xt, m));
+}
+
+Real interp_y_to_corner(const Grid& g, int i, int j, int order, const Real* yt,
+                        const Flags& m) {  // :290-329
+  if (order >= 7) {
+    Real e[8];
+    for (int q = 0; q < 8; ++q) e[q] = table_at(g, i - 4 + q, j, yt, m);
+    return edge7(e);
+  }
+  if (order >= 5) {
+    Real e[6];
+    for (int q = 0; q < 6; ++q) e[q] = table_at(g, i - 3 + q, j, yt, m);
+    return edge5(e);
+  }
+  return scaled_face_average(table_at(g, i - 1, j, yt, m), table_at(g, i, j, yt, m));
+}
+
+Real physical_ez(const State& u) {  // :414-420 (zero background)
+  const Real vx = u.mx / u.rho;
+  const Real vy = u.my / u.rho;
+  return product_sum4(vy, u.bx, vy, Real{0}, vx, -u.by, vx, -Real{0});
+}
+
+Real cell_average_ez(const Grid& g, int i, int j, int order, Real gamma, const Field& u) {
+  // :457-632
+  const int co = order <= 1 ? 1 : 0;
+  const Real fallback = physical_ez(load_cell_state(g, u, i, j, co));
+  if (order != 5 && order != 7) return fallback;
+  const int half = order == 5 ? 2 : 3;
+  if (i - half < -g.ng || i + half > g.nx + g.ng - 1 || j - half < -g.ng ||
+      j + half > g.ny + g.ng - 1)
+    return fallback;
+  const int count = order == 5 ? 3 : 4;
+  const int taps = 2 * half + 1;
+  Real bx_block[7][7], by_block[7][7];
+  for (int a = 0; a < taps; ++a)
+    for (int b = 0; b < taps; ++b) {
+      bx_block[a][b] = cell_bx(g, u.c[5], i + a - half, j + b - half, 0);
+      by_block[a][b] = cell_by(g, u.c[6], i + a - half, j + b - half, 0);
+    }
+  const int raw_index[6] = {0, 1, 2, 3, 4, 7};
+  Real point[4][4];
+  for (int qx = 0; qx < count; ++qx) {
+    Real rows[8][7];  // 0..4 raw rho..energy, 5 bx, 6 by, 7 bz
+    for (int k = 0; k < taps; ++k) {
+      const int oy = k - half;
+      Real samples[7];
+      for (int c = 0; c < 6; ++c) {
+        for (int t = 0; t < taps; ++t)
+          samples[t] = u.c[raw_index[c]][g.index(i + t - half, j + oy)];
+        rows[c == 5 ? 7 : c][k] = point_value(order, qx, samples);
+      }
+      for (int t = 0; t < taps; ++t) samples[t] = bx_block[t][k];
+      rows[5][k] = point_value(order, qx, samples);
+      for (int t = 0; t < taps; ++t) samples[t] = by_block[t][k];
+      rows[6][k] = point_value(order, qx, samples);
+    }
+    for (int qy = 0; qy < count; ++qy) {
+      State st;
+      for (int c = 0; c < 8; ++c) set_comp(st, c, point_value(order, qy, rows[c]));
+      if (!state_is_admissible(st, gamma)) return fallback;
+      point[qy][qx] = physical_ez(st);
+    }
+  }
+  const Real* w = order == 5 ? kMp5Gauss : kMp7Gauss;
+  Real row[4];
+  for (int qy = 0; qy < count; ++qy) row[qy] = weighted_value(w, point[qy], count);
+  return weighted_value(w, row, count);
+}
+
+Real cell_emf_to_corner(const Grid& g, int i, int j, int order, const Real* cell_ez,
+                        const Flags& m) {  // :703-765
+  auto value = [&](int ii, int jj) { return table_at(g, ii, jj, cell_ez, m); };
+  if (order >= 7) {
+    Real line[8];
+    for (int q = 0; q < 8; ++q) {
+      Real e[8];
+      for (int t = 0; t < 8; ++t) e[t] = value(i - 4 + t, j - 4 + q);
+      line[q] = edge7(e);
+    }
+    return edge7(line);
+  }
+  if (order >= 5) {
+    Real line[6];
+    for (int q = 0; q < 6; ++q) {
+      Real e[6];
+      for (int t = 0; t < 6; ++t) e[t] = value(i - 3 + t, j - 3 + q);
+      line[q] = edge5(e);
+    }
+    return edge5(line);
+  }
+  const Real lo = scaled_face_average(value(i - 1, j - 1), value(i, j - 1));
+  const Real hi = scaled_face_average(value(i - 1, j), value(i, j));
+  return scaled_face_average(lo, hi);
+}
+
+bool cell_emf_is_constant_along_axis(const Grid& g, int i, int j, int order,
+                                     const Real* cell_ez, const Flags& m, int axis) {
+  // :770-809
+  const int rlo = order >= 7 ? -4 : order >= 5 ? -3 : -1;
+  const int rhi = order >= 7 ? 3 : order >= 5 ? 2 : 0;
+  auto value = [&](int ii, int jj) { return table_at(g, ii, jj, cell_ez, m); };
+  for (int varying = rlo; varying <= rhi; ++varying) {
+    const Real reference = axis == 0 ? value(i + rlo, j + varying) : value(i + varying, j + rlo);
+    for (int inv = rlo + 1; inv <= rhi; ++inv) {
+      const Real candidate = axis == 0 ? value(i + inv, j + varying) : value(i + varying, j + inv);
+      if (candidate != reference) return false;
+    }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// CFL (src/backend/hip/mhd/mhd_reduce.hip:219-502).
+
+Real product3_quotient2(Real a, Real b, Real c, Real d0, Real d1) {  // :219-238
+  if (!std::isfinite(a) || !std::isfinite(b) || !std::isfinite(c) || !std::isfinite(d0) ||
+      !std::isfinite(d1) || d0 == Real{0} || d1 == Real{0})
+    return (((a * b) * c) / d0) / d1;
+  if (a == Real{0} || b == Real{0} || c == Real{0}) return Real{0};
+  return (((a * b) * c) / d0) / d1;
+}
+
+struct SpeedParts {
+  Real velocity, fast;
+};
+
+SpeedParts face_state_speed_parts(const State& s, int dir, Real gamma) {  // :255-320
+  const Real p = pressure(s, gamma);
+  if (!(std::isfinite(s.rho) && s.rho > Real{0} && std::isfinite(s.mx) &&
+        std::isfinite(s.my) && std::isfinite(s.mz) && std::isfinite(s.energy) &&
+        std::isfinite(s.bx) && std::isfinite(s.by) && std::isfinite(s.bz) &&
+        std::isfinite(p) && p > Real{0}))
+    return {kInf, kInf};
+  const Real sqrt_rho = std::sqrt(s.rho);
+  const Real sound = product3_quotient2(std::sqrt(gamma), std::sqrt(p), Real{1}, sqrt_rho, Real{1});
+  const Real one[2] = {1, 1};
+  const Real sr[2] = {sqrt_rho, sqrt_rho};
+  const Real ax_a[2] = {s.bx, Real{0}}, ay_a[2] = {s.by, Real{0}}, az_a[2] = {s.bz, Real{0}};
+  const Real ax = product_quotient_sum(ax_a, one, sr, one, 2);
+  const Real ay = product_quotient_sum(ay_a, one, sr, one, 2);
+  const Real az = product_quotient_sum(az_a, one, sr, one, 2);
+  if (!(std::isfinite(sound) && std::isfinite(ax) && std::isfinite(ay) && std::isfinite(az)))
+    return {kInf, kInf};
+  const Real alfven = scaled_norm3(ax, ay, az);
+  const Real alfven_normal = std::fabs(dir == 0 ? ax : ay);
+  const Real scale = std::max(sound, alfven);
+  Real fast = Real{0};
+  if (scale > Real{0}) {
+    const Real an = sound / scale;
+    const Real bn = alfven / scale;
+    const Real can = alfven_normal / scale;
+    const Real a2n = an * an;
+    const Real b2n = bn * bn;
+    const Real sum = a2n + b2n;
+    Real disc = sum * sum - Real{4} * a2n * can * can;
+    if (disc < Real{0}) disc = Real{0};
+    fast = scale * std::sqrt(Real{0.5} * (sum + std::sqrt(disc)));
+  }
+  const Real momentum = std::fabs(dir == 0 ? s.mx : s.my);
+  const Real va[1] = {momentum}, vb[1] = {1}, vd0[1] = {s.rho}, vd1[1] = {1};
+  const Real velocity = product_quotient_sum(va, vb, vd0, vd1, 1);
+  if (!(std::isfinite(velocity) && std::isfinite(fast))) return {kInf, kInf};
+  return {velocity, fast};
+}
+
+Real interface_speed(const SpeedParts& l, const SpeedParts& r, int order) {  // :327-337
+  const Real velocity = std::max(l.velocity, r.velocity);
+  const Real fast = std::max(l.fast, r.fast);
+  const Real rate = order <= 1 ? std::max(l.velocity + l.fast, r.velocity + r.fast)
+                               : velocity + fast;
+  return std::isfinite(rate) ? rate : kInf;
+}
+
+Real scaled_face_rate(Real speed, Real w, Real spacing) {  // :343-352
+  if (!(std::isfinite(speed) && speed >= Real{0} && std::isfinite(w) && w > Real{0} &&
+        std::isfinite(spacing) && spacing > Real{0}))
+    return kInf;
+  return product3_quotient2(speed, w, Real{1}, spacing, Real{1});
+}
+
+Real reconstructed_face_rate(const Grid& g, const Flags& flags, Real gamma, int order, int dir,
+                             int fi, int fj, Real w, Real spacing, const Iface& v) {
+  // :360-410
+  if (w == Real{0}) return Real{0};
+  if (dir == 0 && fi == g.nx && flags.side[0] == 0 && flags.side[1] == 0) fi = 0;
+  if (dir == 1 && fj == g.ny && flags.side[2] == 0 && flags.side[3] == 0) fj = 0;
+  const std::size_t k = g.index(fi, fj);
+  const State bl = load_side(v, k, true);
+  const State br = load_side(v, k, false);
+  Real maximum = scaled_face_rate(interface_speed(face_state_speed_parts(bl, dir, gamma),
+                                                  face_state_speed_parts(br, dir, gamma), order),
+                                  w, spacing);
+  if (order != 5 && order != 7) return maximum;
+  const int count = order == 5 ? 3 : 4;
+  for (int q = 0; q < count; ++q) {
+    const State l = point_interface_state(g, dir, fi, fj, order, q, v, true);
+    const State r = point_interface_state(g, dir, fi, fj, order, q, v, false);
+    if (!state_is_admissible(l, gamma) || !state_is_admissible(r, gamma)) return maximum;
+    maximum = std::max(maximum, scaled_face_rate(interface_speed(face_state_speed_parts(l, dir, gamma),
+                                                                 face_state_speed_parts(r, dir, gamma),
+                                                                 order),
+                                                 w, spacing));
+  }
+  return maximum;
+}
+
+Real convex_internal_energy(const State& a, const State& b, Real theta) {  // mhd_reduce.hip:99-123
+  const Real omt = Real{1} - theta;
+  const Real rho = omt * a.rho + theta * b.rho;
+  if (!(rho > Real{0}) || !std::isfinite(rho)) return -kInf;
+  const Real mx = omt * a.mx + theta * b.mx;
+  const Real my = omt * a.my + theta * b.my;
+  const Real mz = omt * a.mz + theta * b.mz;
+  const Real e = omt * a.energy + theta * b.energy;
+  const Real bx = omt * a.bx + theta * b.bx;
+  const Real by = omt * a.by + theta * b.by;
+  const Real bz = omt * a.bz + theta * b.bz;
+  const Real kinetic = kinetic_from_momentum(mx, my, mz, rho);
+  const Real magnetic = half_squared_norm3(bx, by, bz);
+  const Real q = (e - kinetic) - magnetic;
+  return std::isfinite(q) ? q : -kInf;
+}
+
+Real ulp_of(Real v) {  // include/quasar/numerics/mhd_background_metrics.hpp:105-121
+  if (!std::isfinite(v)) return kInf;
+  const Real m = std::fabs(v);
+  if (m < kMin) return std::numeric_limits<Real>::denorm_min();
+  int e = 0;
+  (void)std::frexp(m, &e);
+  return std::ldexp(Real{0.5}, e - std::numeric_limits<Real>::digits + 1);
+}
+
+bool abs_le_pow2(Real lhs, Real rhs, int shift) {  // mhd_background_metrics.hpp:189-203
+  if (!(std::isfinite(lhs) && std::isfinite(rhs))) return false;
+  if (lhs == Real{0}) return true;
+  if (rhs == Real{0}) return false;
+  return std::fabs(lhs) <= std::ldexp(std::fabs(rhs), shift);
+}
+
+// Philox-4x32-10 round constants (Salmon et al. 2011, Random123).
+constexpr std::uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr std::uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr std::uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr std::uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+}  // namespace
+
+// ===========================================================================
+// Public kernels.
+
+Real transverse_point_value(int order, int node, const Real* samples) {
+  return point_value(order, node, samples);
+}
+
+Real transverse_gauss_average(int order, const Real* samples) {
+  return weighted_value(order == 5 ? kMp5Gauss : kMp7Gauss, samples, order == 5 ? 3 : 4);
+}
+
+Real cell_bx(const Grid& g, const Real* bx_face, int i, int j, int co) {  // mhd_staggering.hpp:262-283
+  if (co == 1) return low_order_face_to_cell_average(g, bx_face, 0, i, j);
+  return face_samples_to_cell_average(g, bx_face, 0, i, j);
+}
+
+Real cell_by(const Grid& g, const Real* by_face, int i, int j, int co) {
+  if (co == 1) return low_order_face_to_cell_average(g, by_face, 1, j, i);
+  return face_samples_to_cell_average(g, by_face, 1, j, i);
+}
+
+State load_cell_state(const Grid& g, const Field& u, int i, int j, int co) {  // :387-404
+  const std::size_t k = g.index(i, j);
+  State s;
+  s.rho = u.c[0][k];
+  s.mx = u.c[1][k];
+  s.my = u.c[2][k];
+  s.mz = u.c[3][k];
+  s.energy = u.c[4][k];
+  s.bx = cell_bx(g, u.c[5], i, j, co);
+  s.by = cell_by(g, u.c[6], i, j, co);
+  s.bz = u.c[7][k];
+  return s;
+}
+
+void reconstruct(const Grid& g, int dir, int order, const Flags& flags, Real gamma,
+                 const Field& u, Iface& out, Counters* counters) {  // mhd_reconstruct.hip:465-670
+  const int di = (dir == 0) ? 1 : 0;
+  const int dj = (dir == 0) ? 0 : 1;
+  const int side_lo = (dir == 0) ? flags.side[0] : flags.side[2];
+  const int side_hi = (dir == 0) ? flags.side[1] : flags.side[3];
+  const int i_lo = dir == 0 ? 0 : -g.ng;
+  const int i_hi = dir == 0 ? g.nx : g.nx + g.ng - 1;
+  const int j_lo = dir == 0 ? -g.ng : 0;
+  const int j_hi = dir == 0 ? g.ny + g.ng - 1 : g.ny;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = j_lo; j <= j_hi; ++j) {
+    for (int i = i_lo; i <= i_hi; ++i) {
+      State ul, ur;
+      if (order <= 1) {
+        ul = load_cell_state(g, u, i - di, j - dj, 1);
+        ur = load_cell_state(g, u, i, j, 1);
+      } else if (order != 2) {
+        const State cL = load_cell_state(g, u, i - di, j - dj, 0);
+        const State cR = load_cell_state(g, u, i, j, 0);
+        State ref;
+        for (int c = 0; c < 8; ++c)
+          set_comp(ref, c, Real{0.5} * comp(cL, c) + Real{0.5} * comp(cR, c));
+        const State cref = rotate(ref, dir);
+        Eigen eig;
+        eigen_build(cref, gamma, eig);
+        const int half = (order == 5) ? 2 : 3;
+        const int uw = 2 * half + 2;
+        Real chars[8][7];
+        for (int uu = 0; uu < uw; ++uu) {
+          const int off = -1 - half + uu;
+          const State loaded = load_cell_state(g, u, i + off * di, j + off * dj, 0);
+          State delta = rotate(loaded, dir);
+          for (int c = 0; c < 8; ++c) set_comp(delta, c, comp(delta, c) - comp(cref, c));
+          to_char(delta, eig, chars[uu]);
+        }
+        const int width = 2 * half + 1;
+        State faces[2];
+        for (int side = 0; side < 2; ++side) {
+          const bool right = side == 1;
+          const int base = right ? 1 : 0;
+          Real wf[7];
+          for (int k = 0; k < 7; ++k) {
+            Real scalar[7];
+            for (int s = 0; s < width; ++s) {
+              const int src = right ? (width - 1 - s) : s;
+              scalar[s] = chars[base + src][k];
+            }
+            int clipped = 0;
+            wf[k] = half == 2 ? mp5_reconstruct(scalar, &clipped) : mp7_reconstruct(scalar, &clipped);
+            if (counters && clipped) {
+#pragma omp atomic
+      counters->mp_clips++;
+      }
+          }
+          const State d = from_char(wf, eig);
+          State s;
+          for (int c = 0; c < 8; ++c) set_comp(s, c, comp(cref, c) + comp(d, c));
+          faces[side] = rotate(s, dir);
+        }
+        ul = faces[0];
+        ur = faces[1];
+        if (!reconstructed_state_admissible(ul, gamma) || !reconstructed_state_admissible(ur, gamma)) {
+          if (counters) {
+#pragma omp atomic
+      counters->muscl_fallbacks++;
+      }
+          muscl_reconstruct_face(g, dir, i, j, gamma, side_lo, side_hi, u, ul, ur);
+        }
+      } else {
+        muscl_reconstruct_face(g, dir, i, j, gamma, side_lo, side_hi, u, ul, ur);
+      }
+      const Real bn = (dir == 0 ? u.c[5] : u.c[6])[g.index(i, j)];
+      set_normal_b(ul, dir, bn);
+      set_normal_b(ur, dir, bn);
+      if (!reconstructed_state_admissible(ul, gamma) || !reconstructed_state_admissible(ur, gamma)) {
+        if (counters) {
+#pragma omp atomic
+      counters->pc_fallbacks++;
+      }
+        const int fco = (order <= 1) ? 1 : 0;
+        ul = load_cell_state(g, u, i - di, j - dj, fco);
+        ur = load_cell_state(g, u, i, j, fco);
+        set_normal_b(ul, dir, bn);
+        set_normal_b(ur, dir, bn);
+      }
+      if (!reconstructed_state_admissible(ul, gamma) || !reconstructed_state_admissible(ur, gamma)) {
+        if (counters) {
+#pragma omp atomic
+      counters->sentinels++;
+      }
+        const Real nan = std::numeric_limits<Real>::quiet_NaN();
+        ul = State{nan, nan, nan, nan, nan, nan, nan, nan};
+        ur = ul;
+      }
+      const std::size_t k = g.index(i, j);
+      for (int c = 0; c < 8; ++c) {
+        out.c[c][k] = comp(ul, c);
+        out.c[8 + c][k] = comp(ur, c);
+      }
+    }
+  }
+}
+
+void face_flux(const Grid& g, int dir, int order, int hll_only, const Flags& flags, Real gamma,
+               const Iface& in, Field& flux, Counters* counters) {  // mhd_riemann.hip:53-136
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 0; j <= g.ny; ++j) {
+    for (int i = 0; i <= g.nx; ++i) {
+      int si = i, sj = j;
+      if (dir == 0 && i == g.nx && flags.side[0] == 0 && flags.side[1] == 0) si = 0;
+      if (dir == 1 && j == g.ny && flags.side[2] == 0 && flags.side[3] == 0) sj = 0;
+      const Flux f = average_face_flux(g, dir, si, sj, order, hll_only, gamma, in, counters);
+      const std::size_t k = g.index(i, j);
+      for (int c = 0; c < 8; ++c) flux.c[c][k] = comp(f, c);
+    }
+  }
+}
+
+void flux_difference(const Grid& g, int dir, const Field& flux, Field& dudt) {  // mhd_update.hip:21-105
+  const int di = (dir == 0) ? 1 : 0;
+  const int dj = (dir == 0) ? 0 : 1;
+  const Real sp = (dir == 0) ? g.dx() : g.dy();
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < g.ny; ++j)
+    for (int i = 0; i < g.nx; ++i) {
+      const std::size_t k = g.index(i, j);
+      const std::size_t kp = g.index(i + di, j + dj);
+      for (int c = 0; c < 8; ++c) {
+        const Real a[2] = {flux.c[c][kp], -flux.c[c][k]};
+        const Real one[2] = {1, 1};
+        const Real spacing[2] = {sp, sp};
+        dudt.c[c][k] += -product_quotient_sum(a, one, spacing, one, 2);
+      }
+    }
+}
+
+void ct_emf(const Grid& g, int order, int hll_only, const Flags& m, Real gamma, const Field& u,
+            const Iface& ifx, const Iface& ify, Real* cell_ez, Real* xface_ez, Real* yface_ez,
+            int* xnj, int* ynj, Real* ez_edge) {  // mhd_ct.hip:330-359, 634-701, 811-878
+  const bool phys_x_high = m.side[1] != 0 && m.side[1] != 4;
+  const bool phys_y_high = m.side[3] != 0 && m.side[3] != 4;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 0; j <= g.ny; ++j)
+    for (int i = 0; i <= g.nx; ++i) {
+      if ((i == g.nx && !phys_x_high) || (j == g.ny && !phys_y_high)) continue;
+      cell_ez[g.index(i, j)] = cell_average_ez(g, i, j, order, gamma, u);
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 0; j <= g.ny; ++j)
+    for (int i = 0; i <= g.nx; ++i) {
+      const std::size_t k = g.index(i, j);
+      if (j < g.ny || (j == g.ny && phys_y_high)) {
+        xface_ez[k] = -average_face_flux(g, 0, i, j, order, hll_only, gamma, ifx, nullptr).by;
+        xnj[k] = face_problem_has_no_jump(g, 0, i, j, order, ifx, m) ? 1 : 0;
+      }
+      if (i < g.nx || (i == g.nx && phys_x_high)) {
+        yface_ez[k] = average_face_flux(g, 1, i, j, order, hll_only, gamma, ify, nullptr).bx;
+        ynj[k] = face_problem_has_no_jump(g, 1, i, j, order, ify, m) ? 1 : 0;
+      }
+    }
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j <= g.ny; ++j)
+    for (int i = 0; i <= g.nx; ++i) {
+      const std::size_t k = g.index(i, j);
+      const bool conducting = (i == 0 && (m.side[0] == 2 || m.side[0] == 3)) ||
+                              (i == g.nx && m.side[1] == 2) || (j == 0 && m.side[2] == 2) ||
+                              (j == g.ny && m.side[3] == 2);
+      if (conducting) {
+        ez_edge[k] = Real{0};
+        continue;
+      }
+      const Real ex = interp_x_to_corner(g, i, j, order, xface_ez, m);
+      const Real ey = interp_y_to_corner(g, i, j, order, yface_ez, m);
+      const Real centered = cell_emf_to_corner(g, i, j, order, cell_ez, m);
+      const bool x_only = directional_corner_stencil_has_no_jump(g, 1, i, j, order, ynj, m) &&
+                          cell_emf_is_constant_along_axis(g, i, j, order, cell_ez, m, 1);
+      const bool y_only = directional_corner_stencil_has_no_jump(g, 0, i, j, order, xnj, m) &&
+                          cell_emf_is_constant_along_axis(g, i, j, order, cell_ez, m, 0);
+      if (x_only && !y_only)
+        ez_edge[k] = ex;
+      else if (y_only && !x_only)
+        ez_edge[k] = ey;
+      else
+        ez_edge[k] = product_sum3(ex, Real{1}, ey, Real{1}, centered, Real{-1});
+    }
+}
+
+void emf_curl_rate(const Grid& g, const Real* ez, Field& dudt) {  // mhd_ct.hip:893-931
+  const Real one[2] = {1, 1};
+  for (int j = 0; j <= g.ny; ++j)
+    for (int i = 0; i <= g.nx; ++i) {
+      const std::size_t k = g.index(i, j);
+      if (j < g.ny) {
+        const Real a[2] = {-ez[g.index(i, j + 1)], ez[k]};
+        const Real d0[2] = {g.dy(), g.dy()};
+        dudt.c[5][k] = product_quotient_sum(a, one, d0, one, 2);
+      }
+      if (i < g.nx) {
+        const Real a[2] = {ez[g.index(i + 1, j)], -ez[k]};
+        const Real d0[2] = {g.dx(), g.dx()};
+        dudt.c[6][k] = product_quotient_sum(a, one, d0, one, 2);
+      }
+    }
+}
+
+namespace {
+Real rk_weighted_sum(Real a, Real un, Real b, Real us, Real c, Real d) {  // mhd_update.hip:113-135
+  const bool ua = a != Real{0}, ub = b != Real{0}, uc = c != Real{0};
+  if (ua) {
+    if (ub) return uc ? product_sum3(a, un, b, us, c, d) : product_sum2(a, un, b, us);
+    return uc ? product_sum2(a, un, c, d) : a * un;
+  }
+  if (ub) return uc ? product_sum2(b, us, c, d) : b * us;
+  return uc ? c * d : Real{0};
+}
+}  // namespace
+
+void rk_stage(const Grid& g, Real a, Real b, Real c, const Field& un, const Field& us,
+              const Field& dudt, Field& out) {  // mhd_update.hip:142-206
+  const bool copy = a == Real{1} && b == Real{0} && c == Real{0};
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j <= g.ny; ++j)
+    for (int i = 0; i <= g.nx; ++i) {
+      const bool cell = i < g.nx && j < g.ny;
+      const bool bxp = j < g.ny;
+      const bool byp = i < g.nx;
+      const std::size_t k = g.index(i, j);
+      for (int comp_i = 0; comp_i < 8; ++comp_i) {
+        const bool active = comp_i == 5 ? bxp : comp_i == 6 ? byp : cell;
+        if (!active) continue;
+        out.c[comp_i][k] = copy ? un.c[comp_i][k]
+                                : rk_weighted_sum(a, un.c[comp_i][k], b, us.c[comp_i][k], c,
+                                                  dudt.c[comp_i][k]);
+      }
+    }
+}
+
+void fill_ghosts(const Grid& g, const Flags& fluid, const Flags& field, Field& u) {
+  // src/backend/hip/mhd/mhd_boundary_hip.hip:66-219; src/physics/mhd/mhd_boundary.cpp
+  auto ghost_indices = [](int n, int layer, bool lo, int mode, int& gi, int& si) {
+    gi = lo ? -layer : (n - 1 + layer);
+    if (mode == 0)
+      si = lo ? (n - layer) : (layer - 1);
+    else if (mode == 1)
+      si = lo ? 0 : (n - 1);
+    else
+      si = lo ? (layer - 1) : (n - layer);
+  };
+  for (int side = 0; side < 4; ++side) {
+    const bool x_side = side < 2;
+    const bool lo = side == 0 || side == 2;
+    const int line_lo = -g.ng;
+    const int line_hi = x_side ? g.ny + g.ng : g.nx + g.ng;
+    const int n = x_side ? g.nx : g.ny;
+    // fluid
+    {
+      const int mode = fluid.side[side];
+      const bool active = !(mode == 4 || (mode == 3 && side != 0));
+      const Real s_normal = (mode == 2 || mode == 3) ? Real{-1} : Real{1};
+      if (active)
+        for (int layer = 1; layer <= g.ng; ++layer)
+          for (int line = line_lo; line < line_hi; ++line) {
+            int gi, si;
+            ghost_indices(n, layer, lo, mode, gi, si);
+            const std::size_t kd = x_side ? g.index(gi, line) : g.index(line, gi);
+            const std::size_t ks = x_side ? g.index(si, line) : g.index(line, si);
+            u.c[0][kd] = u.c[0][ks];
+            u.c[1][kd] = (x_side ? s_normal : Real{1}) * u.c[1][ks];
+            u.c[2][kd] = (x_side ? Real{1} : s_normal) * u.c[2][ks];
+            u.c[3][kd] = (x_side && mode == 3 ? Real{-1} : Real{1}) * u.c[3][ks];
+            u.c[4][kd] = u.c[4][ks];
+          }
+    }
+    // field
+    {
+      const int mode = field.side[side];
+      const bool active = !(mode == 4 || (mode == 3 && side != 0));
+      const Real s_normal = (mode == 2 || mode == 3) ? Real{-1} : Real{1};
+      if (active)
+        for (int layer = 1; layer <= g.ng; ++layer)
+          for (int line = line_lo; line < line_hi; ++line) {
+            int gi, si;
+            ghost_indices(n, layer, lo, mode, gi, si);
+            if (x_side) {
+              const int j = line;
+              const std::size_t kd = g.index(gi, j), ks = g.index(si, j);
+              u.c[6][kd] = u.c[6][ks];
+              u.c[7][kd] = (mode == 3 ? Real{-1} : Real{1}) * u.c[7][ks];
+              Real* bx = u.c[5];
+              if (mode == 2 || mode == 3) {
+                if (lo) {
+                  bx[g.index(0, j)] = Real{0};
+                  bx[g.index(-layer, j)] = -bx[g.index(layer, j)];
+                } else if (layer == 1) {
+                  bx[g.index(g.nx, j)] = Real{0};
+                } else {
+                  const int off = layer - 1;
+                  bx[g.index(g.nx + off, j)] = -bx[g.index(g.nx - off, j)];
+                }
+              } else if (mode == 1 && !lo) {
+                if (layer > 1) bx[g.index(g.nx - 1 + layer, j)] = bx[g.index(g.nx, j)];
+              } else {
+                bx[kd] = s_normal * bx[ks];
+              }
+            } else {
+              const int i = line;
+              const std::size_t kd = g.index(i, gi), ks = g.index(i, si);
+              u.c[5][kd] = u.c[5][ks];
+              u.c[7][kd] = u.c[7][ks];
+              Real* by = u.c[6];
+              if (mode == 2) {
+                if (lo) {
+                  by[g.index(i, 0)] = Real{0};
+                  by[g.index(i, -layer)] = -by[g.index(i, layer)];
+                } else if (layer == 1) {
+                  by[g.index(i, g.ny)] = Real{0};
+                } else {
+                  const int off = layer - 1;
+                  by[g.index(i, g.ny + off)] = -by[g.index(i, g.ny - off)];
+                }
+              } else if (mode == 1 && !lo) {
+                if (layer > 1) by[g.index(i, g.ny - 1 + layer)] = by[g.index(i, g.ny)];
+              } else {
+                by[kd] = s_normal * by[ks];
+              }
+            }
+          }
+    }
+  }
+}
+
+Real cfl_max_rate(const Grid& g, int order, const Flags& flags, Real gamma, const Iface& ifx,
+                  const Iface& ify) {  // mhd_reduce.hip:418-502, 716-734
+  Real best = Real{0};
+  const Real half_weight = Real{0.5} * Real{1};
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : best)
+  for (int j = 0; j < g.ny; ++j)
+    for (int i = 0; i < g.nx; ++i) {
+      const Real ax_lo = reconstructed_face_rate(g, flags, gamma, order, 0, i, j, half_weight, g.dx(), ifx);
+      const Real ax_hi = reconstructed_face_rate(g, flags, gamma, order, 0, i + 1, j, half_weight, g.dx(), ifx);
+      const Real ay_lo = reconstructed_face_rate(g, flags, gamma, order, 1, i, j, half_weight, g.dy(), ify);
+      const Real ay_hi = reconstructed_face_rate(g, flags, gamma, order, 1, i, j + 1, half_weight, g.dy(), ify);
+      const Real radial = ax_lo + ax_hi;
+      const Real axial = ay_lo + ay_hi;
+      Real local = radial + axial;
+      if (!std::isfinite(local))
+        local = kInf;
+      else
+        local = std::max(Real{0}, local);
+      if (local > best) best = local;
+    }
+  return best;
+}
+
+Real admissible_fraction(const Grid& g, const Field& base, const Field& cand, Real rho_floor,
+                         Real q_floor, int co) {  // mhd_reduce.hip:130-207, 741-770
+  Real best = Real{1};
+#pragma omp parallel for schedule(static) reduction(min : best)
+  for (int j = 0; j < g.ny; ++j)
+    for (int i = 0; i < g.nx; ++i) {
+      const State a = load_cell_state(g, base, i, j, co);
+      const State b = load_cell_state(g, cand, i, j, co);
+      const Real qa = convex_internal_energy(a, b, Real{0});
+      Real local;
+      if (!(a.rho > rho_floor) || !(qa > q_floor) || !std::isfinite(a.rho) || !std::isfinite(qa)) {
+        local = Real{0};
+      } else if (!std::isfinite(b.rho) || !std::isfinite(b.mx) || !std::isfinite(b.my) ||
+                 !std::isfinite(b.mz) || !std::isfinite(b.energy) || !std::isfinite(b.bx) ||
+                 !std::isfinite(b.by) || !std::isfinite(b.bz)) {
+        local = Real{0};
+      } else {
+        Real hi = Real{1};
+        if (b.rho <= rho_floor) {
+          const Real denom = a.rho - b.rho;
+          hi = (denom > Real{0}) ? (a.rho - rho_floor) / denom : Real{0};
+          if (hi < Real{0}) hi = Real{0};
+          if (hi > Real{1}) hi = Real{1};
+        }
+        if (convex_internal_energy(a, b, hi) > q_floor) {
+          local = hi;
+        } else {
+          Real lo = Real{0};
+          for (int it = 0; it < 56; ++it) {
+            const Real mid = Real{0.5} * (lo + hi);
+            if (convex_internal_energy(a, b, mid) > q_floor)
+              lo = mid;
+            else
+              hi = mid;
+          }
+          local = lo;
+        }
+      }
+      const Real mapped = std::isfinite(local) ? local : Real{0};
+      if (mapped < best) best = mapped;
+    }
+  return std::fmax(Real{0}, std::fmin(Real{1}, best));
+}
+
+Real divb_linf(const Grid& g, const Field& u) {  // mhd_reduce.hip:505-549
+  Real best = Real{0};
+  for (int j = 0; j < g.ny; ++j)
+    for (int i = 0; i < g.nx; ++i) {
+      const Real a[4] = {u.c[5][g.index(i + 1, j)], -u.c[5][g.index(i, j)],
+                         u.c[6][g.index(i, j + 1)], -u.c[6][g.index(i, j)]};
+      const Real b[4] = {1, 1, 1, 1};
+      const Real d0[4] = {g.dx(), g.dx(), g.dy(), g.dy()};
+      const Real div = product_quotient_sum(a, b, d0, b, 4);
+      const Real local = std::isfinite(div) ? std::fabs(div) : kInf;
+      if (local > best) best = local;
+    }
+  return best;
+}
+
+Real divb_relative_linf(const Grid& g, const Field& u, bool solver_owned) {
+  // mhd_reduce.hip:595-714; mhd_background_metrics.hpp:53-131
+  Real res_max = Real{0}, scale_max = Real{0};
+  bool bad = false;
+  for (int j = 0; j < g.ny; ++j)
+    for (int i = 0; i < g.nx; ++i) {
+      const Real bx_lo = u.c[5][g.index(i, j)], bx_hi = u.c[5][g.index(i + 1, j)];
+      const Real by_lo = u.c[6][g.index(i, j)], by_hi = u.c[6][g.index(i, j + 1)];
+      const Real radial = (bx_hi - bx_lo) / g.dx();
+      const Real axial = (by_hi - by_lo) / g.dy();
+      const Real rr[2] = {ulp_of(bx_hi) / g.dx(), ulp_of(bx_lo) / g.dx()};
+      const Real ar[2] = {ulp_of(by_hi) / g.dy(), ulp_of(by_lo) / g.dy()};
+      const Real radial_roundoff = cr_sum(rr, 2);
+      const Real axial_roundoff = cr_sum(ar, 2);
+      if (!(std::isfinite(radial) && std::isfinite(axial))) {
+        bad = true;
+        continue;
+      }
+      const Real rs[2] = {radial, axial};
+      const Real residual = cr_sum(rs, 2);
+      const Real ms[2] = {std::fabs(radial), std::fabs(axial)};
+      const Real scale = cr_sum(ms, 2);
+      const Real ro[2] = {radial_roundoff, axial_roundoff};
+      const Real roundoff = cr_sum(ro, 2);
+      bool explained;
+      if (solver_owned) {
+        explained = abs_le_pow2(residual, roundoff, 10);
+      } else if (radial == Real{0} || axial == Real{0} ||
+                 std::signbit(radial) == std::signbit(axial)) {
+        explained = false;
+      } else if (!abs_le_pow2(residual, scale, -1)) {
+        explained = false;
+      } else {
+        explained = abs_le_pow2(residual, roundoff, 10);
+      }
+      const Real r = explained ? Real{0} : std::fabs(residual);
+      if (r > res_max) res_max = r;
+      if (scale > scale_max) scale_max = scale;
+    }
+  if (bad) return kInf;
+  if (res_max == Real{0}) return Real{0};
+  if (!(scale_max > Real{0})) return kInf;
+  return res_max / scale_max;
+}
+
+void conserved_totals(const Grid& g, const Field& u, Real* mass, Real* energy) {
+  // src/backend/hip/mhd/mhd_cell_sums.hip:54-163: the exact sum, correctly rounded.
+  std::vector<Real> m, e;
+  m.reserve(static_cast<std::size_t>(g.nx) * g.ny);
+  e.reserve(static_cast<std::size_t>(g.nx) * g.ny);
+  for (int j = 0; j < g.ny; ++j)
+    for (int i = 0; i < g.nx; ++i) {
+      m.push_back(u.c[0][g.index(i, j)]);
+      e.push_back(u.c[4][g.index(i, j)]);
+    }
+  // Pairwise exact accumulation: cr_sum over chunks keeps every partial exact.
+  auto exact_sum = [](const std::vector<Real>& v) {
+    std::vector<Real> partials;
+    for (Real x : v) {
+      if (x == Real{0}) continue;
+      std::size_t i = 0;
+      for (std::size_t p = 0; p < partials.size(); ++p) {
+        Real y = partials[p];
+        if (std::fabs(x) < std::fabs(y)) std::swap(x, y);
+        const Real hi = x + y;
+        const Real lo = y - (hi - x);
+        if (lo != Real{0}) partials[i++] = lo;
+        x = hi;
+      }
+      partials.resize(i);
+      partials.push_back(x);
+    }
+    return cr_sum(partials.data(), static_cast<int>(partials.size()));
+  };
+  *mass = exact_sum(m);
+  *energy = exact_sum(e);
+}
+
+// ---------------------------------------------------------------------------
+// Initial conditions (src/backend/hip/mhd/mhd_initial_conditions.hip:67-342).
+
+void philox4x32_10(std::uint32_t ctr[4], const std::uint32_t key_in[2]) {
+  std::uint32_t key[2] = {key_in[0], key_in[1]};
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      key[0] += kPhiloxW0;
+      key[1] += kPhiloxW1;
+    }
+    const std::uint64_t p0 = static_cast<std::uint64_t>(kPhiloxM0) * ctr[0];
+    const std::uint64_t p1 = static_cast<std::uint64_t>(kPhiloxM1) * ctr[2];
+    const std::uint32_t hi0 = static_cast<std::uint32_t>(p0 >> 32), lo0 = static_cast<std::uint32_t>(p0);
+    const std::uint32_t hi1 = static_cast<std::uint32_t>(p1 >> 32), lo1 = static_cast<std::uint32_t>(p1);
+    const std::uint32_t c1 = ctr[1], c3 = ctr[3];
+    ctr[0] = hi1 ^ c1 ^ key[0];
+    ctr[1] = lo1;
+    ctr[2] = hi0 ^ c3 ^ key[1];
+    ctr[3] = lo0;
+  }
+}
+
+Real philox_uniform_pm1(std::uint64_t seed, std::uint32_t component, std::uint64_t cell) {
+  std::uint32_t ctr[4] = {static_cast<std::uint32_t>(cell), static_cast<std::uint32_t>(cell >> 32),
+                          component, 0u};
+  const std::uint32_t key[2] = {static_cast<std::uint32_t>(seed), static_cast<std::uint32_t>(seed >> 32)};
+  philox4x32_10(ctr, key);
+  const std::uint64_t bits = (static_cast<std::uint64_t>(ctr[1]) << 32) | ctr[0];
+  const Real u = static_cast<Real>(bits >> 11) * 0x1.0p-53;
+  return Real{2} * u - Real{1};
+}
+
+int seed_initial_state(const Grid& g, const IcSpec& s, Field& out) {
+  const Real pi = 3.141592653589793238462643383279502884;
+  int status = 0;
+  std::vector<Real> raw(g.size(), Real{0});
+  for (int j = -g.ng; j < g.ny + g.ng; ++j)
+    for (int i = -g.ng; i < g.nx + g.ng; ++i) {
+      const Real xc = g.xc(i), yc = g.yc(j);
+      Prim w{};
+      Real energy_offset = Real{0};
+      switch (s.kind) {
+        case 0: {  // brio_wu :67-81
+          const Real* side = (xc < s.interface_x) ? s.left : s.right;
+          w.rho = side[0];
+          w.p = side[1];
+          w.vx = side[2];
+          w.vy = side[3];
+          w.vz = side[4];
+          w.bx = side[5];
+          w.by = side[6];
+          w.bz = side[7];
+          break;
+        }
+        case 1: {  // alfven_wave :83-114
+          const Real n = static_cast<Real>(s.wavenumber);
+          const Real k = Real{2} * pi * n / g.lx;
+          const Real phase = k * (xc - g.ox);
+          const Real q = n / static_cast<Real>(g.nx);
+          const Real fv = q == Real{0} ? Real{1} : std::sin(pi * q) / (pi * q);
+          const Real inv_sqrt_rho = std::copysign(s.magnetic_velocity_scale, s.total_b0) / std::sqrt(s.rho);
+          w.rho = s.rho;
+          w.p = s.pressure;
+          w.vx = Real{0};
+          w.vy = -s.amplitude * inv_sqrt_rho * fv * std::sin(phase);
+          w.vz = -s.amplitude * inv_sqrt_rho * fv * std::cos(phase);
+          w.bx = s.b0;
+          w.by = s.amplitude * fv * std::sin(phase);
+          w.bz = s.amplitude * fv * std::cos(phase);
+          const Real sa = s.amplitude * s.magnetic_velocity_scale;
+          energy_offset = sa * sa * (Real{1} - fv * fv);
+          break;
+        }
+        case 2: {  // orszag_tang :116-138 with explicit background density/pressure
+          const Real xi = (xc - g.ox) / g.lx;
+          const Real eta = (yc - g.oy) / g.ly;
+          const Real two_pi = Real{2} * pi;
+          const Real b0 = s.b_uniform[0];
+          w.rho = s.ot_rho != Real{0} ? s.ot_rho : s.gamma * s.gamma;
+          w.p = s.ot_p != Real{0} ? s.ot_p : s.gamma;
+          w.vx = -std::sin(two_pi * eta);
+          w.vy = std::sin(two_pi * xi);
+          w.vz = Real{0};
+          w.bx = -b0 * std::sin(two_pi * eta);
+          w.by = b0 * std::sin(Real{2} * two_pi * xi);
+          w.bz = Real{0};
+          break;
+        }
+        case 3: {  // blast :140-153
+          const Real r = std::hypot(xc - s.center[0], yc - s.center[1]);
+          w.rho = s.rho_ambient;
+          w.p = (r < s.r_in) ? s.p_core : s.p_ambient;
+          w.bx = s.b_uniform[0];
+          w.by = s.b_uniform[1];
+          w.bz = s.b_uniform[2];
+          break;
+        }
+        case 4: {  // rotor :155-188
+          const Real rx = xc - s.center[0];
+          const Real ry = yc - s.center[1];
+          const Real r = std::hypot(rx, ry);
+          Real f = (s.r1 - r) / (s.r1 - s.r0);
+          f = std::fmin(std::fmax(f, Real{0}), Real{1});
+          const bool inside = r <= s.r0;
+          if (inside) f = Real{1};
+          if (r >= s.r1) f = Real{0};
+          Real speed = Real{0};
+          if (inside)
+            speed = s.u0 / s.r0;
+          else if (r < s.r1)
+            speed = f * s.u0 / r;
+          w.rho = s.rho_out + f * (s.rho_in - s.rho_out);
+          w.p = s.p_ambient;
+          w.vx = -speed * ry;
+          w.vy = speed * rx;
+          w.bx = s.b_uniform[0];
+          w.by = s.b_uniform[1];
+          w.bz = s.b_uniform[2];
+          break;
+        }
+        default:
+          return -1;
+      }
+      if (s.noise_amplitude != Real{0}) {
+        int wi = i % g.nx;
+        if (wi < 0) wi += g.nx;
+        int wj = j % g.ny;
+        if (wj < 0) wj += g.ny;
+        const std::uint64_t cell = static_cast<std::uint64_t>(wj) * static_cast<std::uint64_t>(g.nx) +
+                                   static_cast<std::uint64_t>(wi);
+        w.rho = w.rho * (Real{1} + s.noise_amplitude * philox_uniform_pm1(s.noise_seed, 0, cell));
+        w.vx = w.vx + s.noise_amplitude * philox_uniform_pm1(s.noise_seed, 1, cell);
+        w.vy = w.vy + s.noise_amplitude * philox_uniform_pm1(s.noise_seed, 2, cell);
+      }
+      if (!(std::isfinite(w.rho) && w.rho > Real{0})) { status |= 2; continue; }
+      if (!(std::isfinite(w.p) && w.p > Real{0})) { status |= 4; continue; }
+      const Real kinetic = kinetic_from_velocity(w.rho, w.vx, w.vy, w.vz);
+      const Real magnetic = half_squared_norm3(w.bx, w.by, w.bz);
+      const Real total = w.p / (s.gamma - Real{1}) + kinetic + magnetic + energy_offset;
+      const std::size_t k = g.index(i, j);
+      out.c[0][k] = w.rho;
+      out.c[1][k] = w.rho * w.vx;
+      out.c[2][k] = w.rho * w.vy;
+      out.c[3][k] = w.rho * w.vz;
+      out.c[4][k] = total;
+      out.c[5][k] = s.magnetic_scale * w.bx;
+      out.c[6][k] = s.magnetic_scale * w.by;
+      out.c[7][k] = s.magnetic_scale * w.bz;
+      raw[k] = magnetic;
+    }
+  for (int j = -g.ng; j < g.ny + g.ng; ++j)
+    for (int i = -g.ng; i < g.nx + g.ng; ++i) {
+      const std::size_t k = g.index(i, j);
+      const Real bxc = cell_bx(g, out.c[5], i, j, 0);
+      const Real byc = cell_by(g, out.c[6], i, j, 0);
+      const Real collocated = half_squared_norm3(bxc, byc, out.c[7][k]);
+      const Real adjusted = (out.c[4][k] - raw[k]) + collocated;
+      out.c[4][k] = adjusted;
+      State st{out.c[0][k], out.c[1][k], out.c[2][k], out.c[3][k], adjusted, bxc, byc, out.c[7][k]};
+      const Real p = pressure(st, s.gamma);
+      if (!(p > Real{0})) status |= 4;
+    }
+  return status;
+}
+
+}  // namespace oracle

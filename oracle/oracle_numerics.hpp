@@ -1,0 +1,947 @@
+// ORACLE — test infrastructure only. Never linked into the product library.
+//
+// CPU FP64 restatement of the per-state numerics of quasar's 2D ideal-MHD step:
+// exact (correctly rounded) sums, the equation of state, the Suresh-Huynh MP
+// limiter, the 7-wave eigensystem with its characteristic projection, and the
+// Miyoshi-Kusano HLLD / HLL / Lax-Friedrichs fluxes.  Every function names the
+// reference file:line whose arithmetic it restates; the evaluation order of each
+// expression follows the reference so that, compiled with -ffp-contract=off,
+// results are bit-identical to the reference's header-only numerics (checked in
+// tests/test_oracle_vs_reference.py against oracle/_ref, built from the
+// reference headers in place).
+//
+// Range extension: the reference carries products and sums in a
+// mantissa/exponent frame so that intermediates outside binary64's range stay
+// exact.  This restatement evaluates the same correctly rounded results with
+// ordinary binary64 products; the two agree whenever every product is a normal
+// binary64 number, which holds for every state the step produces on the
+// benchmark configurations.  Out-of-range intermediates are not restated.
+
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <limits>
+#include <utility>
+
+namespace oracle {
+
+using Real = double;
+
+struct State {
+  Real rho{}, mx{}, my{}, mz{}, energy{}, bx{}, by{}, bz{};
+};
+struct Prim {
+  Real rho{}, vx{}, vy{}, vz{}, p{}, bx{}, by{}, bz{};
+};
+using Flux = State;
+
+constexpr Real kEps = std::numeric_limits<Real>::epsilon();
+constexpr Real kMin = std::numeric_limits<Real>::min();
+
+// ---------------------------------------------------------------------------
+// Correctly rounded summation of binary64 terms.
+//
+// The reference reduces short lists of products through an exact
+// non-overlapping expansion followed by a half-even final rounding
+// (include/quasar/numerics/scaled_arithmetic.hpp:121-208).  The result is the
+// correctly rounded value of the exact real sum, which is what Shewchuk's
+// partials algorithm with the Hettinger half-even finalisation (the published
+// algorithm behind Python's math.fsum) also returns.  Zero terms are skipped as
+// the reference does (scaled_arithmetic.hpp:126-132), so an exactly cancelling
+// or empty sum is +0.
+inline Real cr_sum(const Real* terms, int n) {
+  Real partials[80];
+  int np = 0;
+  for (int t = 0; t < n; ++t) {
+    Real x = terms[t];
+    if (x == Real{0}) continue;
+    int i = 0;
+    for (int p = 0; p < np; ++p) {
+      Real y = partials[p];
+      if (std::fabs(x) < std::fabs(y)) std::swap(x, y);
+      const Real hi = x + y;
+      const Real lo = y - (hi - x);
+      if (lo != Real{0}) partials[i++] = lo;
+      x = hi;
+    }
+    np = i;
+    partials[np++] = x;
+  }
+  if (np == 0) return Real{0};
+  int n_left = np;
+  Real hi = partials[--n_left];
+  Real lo = Real{0};
+  while (n_left > 0) {
+    const Real x = hi;
+    const Real y = partials[--n_left];
+    hi = x + y;
+    const Real yr = hi - x;
+    lo = y - yr;
+    if (lo != Real{0}) break;
+  }
+  if (n_left > 0 && ((lo < Real{0} && partials[n_left - 1] < Real{0}) ||
+                     (lo > Real{0} && partials[n_left - 1] > Real{0}))) {
+    const Real y = lo * Real{2};
+    const Real x = hi + y;
+    const Real yr = x - hi;
+    if (y == yr) hi = x;
+  }
+  return hi == Real{0} ? Real{0} : hi;
+}
+
+// Sum of rounded products a[k]*b[k]; non-finite operands keep IEEE semantics
+// (scaled_arithmetic.hpp:231-269).
+inline Real product_sum(const Real* a, const Real* b, int n) {
+  Real terms[32];
+  int count = 0;
+  Real nonfinite = Real{0};
+  bool has_nonfinite = false;
+  for (int k = 0; k < n; ++k) {
+    if (!std::isfinite(a[k]) || !std::isfinite(b[k])) {
+      const Real product = a[k] * b[k];
+      nonfinite = has_nonfinite ? nonfinite + product : product;
+      has_nonfinite = true;
+      continue;
+    }
+    if (a[k] == Real{0} || b[k] == Real{0}) continue;
+    terms[count++] = a[k] * b[k];
+  }
+  const Real finite = cr_sum(terms, count);
+  return has_nonfinite ? nonfinite + finite : finite;
+}
+
+inline Real product_sum2(Real a0, Real b0, Real a1, Real b1) {
+  const Real a[2] = {a0, a1};
+  const Real b[2] = {b0, b1};
+  return product_sum(a, b, 2);
+}
+inline Real product_sum3(Real a0, Real b0, Real a1, Real b1, Real a2, Real b2) {
+  const Real a[3] = {a0, a1, a2};
+  const Real b[3] = {b0, b1, b2};
+  return product_sum(a, b, 3);
+}
+inline Real product_sum4(Real a0, Real b0, Real a1, Real b1, Real a2, Real b2,
+                         Real a3, Real b3) {
+  const Real a[4] = {a0, a1, a2, a3};
+  const Real b[4] = {b0, b1, b2, b3};
+  return product_sum(a, b, 4);
+}
+
+// One term a*b/(d0*d1) with the reference's rounding sequence
+// (((a*b)/d0)/d1), each step rounded (scaled_arithmetic.hpp:496-537).
+inline Real product_quotient_term(Real a, Real b, Real d0, Real d1) {
+  return ((a * b) / d0) / d1;
+}
+
+// Sum of product-quotient terms (scaled_arithmetic.hpp:703-724).
+inline Real product_quotient_sum(const Real* a, const Real* b, const Real* d0,
+                                 const Real* d1, int n) {
+  Real terms[32];
+  int count = 0;
+  Real nonfinite = Real{0};
+  bool has_nonfinite = false;
+  for (int k = 0; k < n; ++k) {
+    if (!std::isfinite(a[k]) || !std::isfinite(b[k]) || !std::isfinite(d0[k]) ||
+        !std::isfinite(d1[k]) || d0[k] == Real{0} || d1[k] == Real{0}) {
+      const Real term = ((a[k] * b[k]) / d0[k]) / d1[k];
+      if (!std::isfinite(term)) {
+        nonfinite = has_nonfinite ? nonfinite + term : term;
+        has_nonfinite = true;
+      } else if (term != Real{0}) {
+        terms[count++] = term;
+      }
+      continue;
+    }
+    if (a[k] == Real{0} || b[k] == Real{0}) continue;
+    terms[count++] = product_quotient_term(a[k], b[k], d0[k], d1[k]);
+  }
+  const Real finite = cr_sum(terms, count);
+  return has_nonfinite ? nonfinite + finite : finite;
+}
+
+// Integer-coefficient rational stencil sum_k w_k s_k / D: the numerator is the
+// correctly rounded exact sum of the error-free products w_k*s_k, then divided
+// once by D (include/quasar/physics/mhd/mhd_staggering.hpp:59-70;
+// scaled_arithmetic.hpp:448-477, 486-502).
+inline Real rational_stencil(const Real* w, const Real* s, int n, Real denom) {
+  Real terms[64];
+  int count = 0;
+  bool finite = true;
+  Real nonfinite = Real{0};
+  for (int k = 0; k < n; ++k) {
+    if (!std::isfinite(w[k]) || !std::isfinite(s[k])) {
+      const Real product = w[k] * s[k];
+      nonfinite = finite ? product : nonfinite + product;
+      finite = false;
+      continue;
+    }
+    if (w[k] == Real{0} || s[k] == Real{0}) continue;
+    const Real hi = w[k] * s[k];
+    const Real lo = std::fma(w[k], s[k], -hi);
+    terms[count++] = hi;
+    if (lo != Real{0}) terms[count++] = lo;
+  }
+  const Real numerator = cr_sum(terms, count);
+  if (!finite) return (nonfinite + numerator) / denom;
+  if (numerator == Real{0}) return numerator / denom;
+  return numerator / denom;
+}
+
+// Weighted reduction with the exact constant-sample shortcut
+// (include/quasar/numerics/finite_volume_quadrature.hpp:55-68).
+inline Real weighted_value(const Real* w, const Real* v, int n) {
+  const Real reference = v[n / 2];
+  bool constant = true;
+  Real ones[16];
+  for (int q = 0; q < n; ++q) {
+    constant = constant && v[q] == reference;
+    ones[q] = Real{1};
+  }
+  if (constant) return reference;
+  return product_quotient_sum(w, v, ones, ones, n);
+}
+
+// ---------------------------------------------------------------------------
+// Equation of state (include/quasar/numerics/mhd_state.hpp:86-255).
+inline Real scaled_norm3(Real x, Real y, Real z) {  // mhd_state.hpp:86-94
+  const Real scale = std::fmax(std::fabs(x), std::fmax(std::fabs(y), std::fabs(z)));
+  if (scale == Real{0}) return Real{0};
+  const Real xs = x / scale, ys = y / scale, zs = z / scale;
+  return scale * std::sqrt(xs * xs + ys * ys + zs * zs);
+}
+
+inline Real half_squared_norm3(Real x, Real y, Real z) {  // mhd_state.hpp:99-108
+  const Real scale = std::fmax(std::fabs(x), std::fmax(std::fabs(y), std::fabs(z)));
+  if (scale == Real{0}) return Real{0};
+  const Real xs = x / scale, ys = y / scale, zs = z / scale;
+  const Real ns = xs * xs + ys * ys + zs * zs;
+  return (Real{0.5} * scale) * (scale * ns);
+}
+
+inline Real kinetic_from_momentum(Real mx, Real my, Real mz, Real rho) {  // :112-122
+  const Real scale = std::fmax(std::fabs(mx), std::fmax(std::fabs(my), std::fabs(mz)));
+  if (scale == Real{0}) return Real{0};
+  const Real xs = mx / scale, ys = my / scale, zs = mz / scale;
+  const Real q = scale / std::sqrt(rho);
+  const Real ns = xs * xs + ys * ys + zs * zs;
+  return (Real{0.5} * q) * (q * ns);
+}
+
+inline Real kinetic_from_velocity(Real rho, Real vx, Real vy, Real vz) {  // :125-135
+  const Real scale = std::fmax(std::fabs(vx), std::fmax(std::fabs(vy), std::fabs(vz)));
+  if (scale == Real{0}) return Real{0};
+  const Real xs = vx / scale, ys = vy / scale, zs = vz / scale;
+  const Real q = std::sqrt(rho) * scale;
+  const Real ns = xs * xs + ys * ys + zs * zs;
+  return (Real{0.5} * q) * (q * ns);
+}
+
+// mhd_state.hpp:142-180
+inline void magnetosonic_speeds(Real rho, Real p, Real bx, Real by, Real bz,
+                                int dir, Real gamma, Real& fast, Real& slow,
+                                Real& sound, Real& alfven_normal,
+                                Real& alfven_total) {
+  const Real sqrt_rho = std::sqrt(rho);
+  sound = (p > Real{0} && gamma > Real{0})
+              ? std::sqrt(gamma) * (std::sqrt(p) / sqrt_rho)
+              : Real{0};
+  const Real ax = bx / sqrt_rho, ay = by / sqrt_rho, az = bz / sqrt_rho;
+  alfven_total = scaled_norm3(ax, ay, az);
+  alfven_normal = std::fabs((dir == 0) ? ax : ay);
+  const Real scale = std::fmax(sound, alfven_total);
+  if (!(scale > Real{0})) {
+    fast = Real{0};
+    slow = Real{0};
+    return;
+  }
+  const Real an = sound / scale;
+  const Real bn = alfven_total / scale;
+  const Real can = alfven_normal / scale;
+  const Real a2n = an * an, b2n = bn * bn, can2 = can * can;
+  const Real sum = a2n + b2n;
+  Real disc = sum * sum - Real{4} * a2n * can2;
+  if (disc < Real{0}) disc = Real{0};
+  const Real fast2n = Real{0.5} * (sum + std::sqrt(disc));
+  const Real slow2n = (fast2n > Real{0}) ? (a2n * can2) / fast2n : Real{0};
+  fast = scale * std::sqrt(fast2n);
+  slow = scale * std::sqrt(slow2n);
+}
+
+inline Real pressure(const State& u, Real gamma) {  // mhd_state.hpp:183-189
+  const Real kinetic = kinetic_from_momentum(u.mx, u.my, u.mz, u.rho);
+  const Real magnetic = half_squared_norm3(u.bx, u.by, u.bz);
+  return (gamma - Real{1}) * ((u.energy - kinetic) - magnetic);
+}
+
+inline Prim to_primitive(const State& u, Real gamma) {  // mhd_state.hpp:228-239
+  Prim w;
+  w.rho = u.rho;
+  w.vx = u.mx / u.rho;
+  w.vy = u.my / u.rho;
+  w.vz = u.mz / u.rho;
+  w.bx = u.bx;
+  w.by = u.by;
+  w.bz = u.bz;
+  w.p = pressure(u, gamma);
+  return w;
+}
+
+inline State to_conserved(const Prim& w, Real gamma) {  // mhd_state.hpp:243-255
+  State u;
+  u.rho = w.rho;
+  u.mx = w.rho * w.vx;
+  u.my = w.rho * w.vy;
+  u.mz = w.rho * w.vz;
+  u.bx = w.bx;
+  u.by = w.by;
+  u.bz = w.bz;
+  const Real kinetic = kinetic_from_velocity(w.rho, w.vx, w.vy, w.vz);
+  const Real magnetic = half_squared_norm3(w.bx, w.by, w.bz);
+  u.energy = w.p / (gamma - Real{1}) + kinetic + magnetic;
+  return u;
+}
+
+// Zero-background fast speed; the reference adds the (zero) background to each
+// field component first (mhd_state.hpp:272-291).
+inline Real fast_speed(const State& u, int dir, Real gamma) {
+  const Real p = pressure(u, gamma);
+  Real fast, slow, sound, an, at;
+  magnetosonic_speeds(u.rho, p, u.bx + Real{0}, u.by + Real{0}, u.bz + Real{0},
+                      dir, gamma, fast, slow, sound, an, at);
+  return fast;
+}
+
+// ---------------------------------------------------------------------------
+// MP limiter (include/quasar/numerics/mp_limiter.hpp:32-224).
+inline Real minmod2(Real a, Real b) {  // :32-42
+  if (a == Real{0} || b == Real{0} || ((a < Real{0}) != (b < Real{0}))) return Real{0};
+  return (std::fabs(a) < std::fabs(b)) ? a : b;
+}
+
+inline Real sgn0(Real a) {
+  return (a > Real{0}) ? Real{1} : ((a < Real{0}) ? Real{-1} : Real{0});
+}
+
+inline Real minmod4(Real a, Real b, Real c, Real d) {  // :46-56
+  const Real sa = sgn0(a), sb = sgn0(b), sc = sgn0(c), sd = sgn0(d);
+  if (!(sa == sb && sb == sc && sc == sd) || sa == Real{0}) return Real{0};
+  const Real mag = std::fmin(std::fmin(std::fabs(a), std::fabs(b)),
+                             std::fmin(std::fabs(c), std::fabs(d)));
+  return sa * mag;
+}
+
+inline Real median3(Real a, Real b, Real c) { return a + minmod2(b - a, c - a); }
+
+// Flag bits reported by mp_limit for label generation: 1 = clipped.
+inline Real mp_limit(Real vl, Real vm2, Real vm1, Real v0, Real vp1, Real vp2,
+                     int* clipped = nullptr) {  // :66-148
+  constexpr Real alpha = Real{4};
+  constexpr Real eps = static_cast<Real>(1e-10);
+  const Real d_minus = v0 - vm1;
+  const Real d_plus = vp1 - v0;
+  const Real vmp = v0 + minmod2(d_plus, alpha * d_minus);
+  const Real cfc = vl - v0;
+  const Real cfm = vl - vmp;
+  const bool between = (cfc <= Real{0} && cfm >= Real{0}) ||
+                       (cfc >= Real{0} && cfm <= Real{0});
+  const Real variation = std::fmax(std::fmax(std::fabs(d_minus), std::fabs(d_plus)),
+                                   std::fmax(std::fabs(cfc), std::fabs(cfm)));
+  const bool within = variation > Real{0} &&
+                      (std::fabs(cfc) / variation) * (std::fabs(cfm) / variation) <= eps;
+  if (between || within) return vl;
+  if (clipped) *clipped = 1;
+  const Real dm = (vm2 - vm1) - (vm1 - v0);
+  const Real d0 = (vm1 - v0) - (v0 - vp1);
+  const Real dp = (v0 - vp1) - (vp1 - vp2);
+  const Real cs = std::fmax(std::fabs(dm), std::fmax(std::fabs(d0), std::fabs(dp)));
+  Real djph = Real{0}, djmh = Real{0};
+  if (cs > Real{0}) {
+    const Real dms = dm / cs, d0s = d0 / cs, dps = dp / cs;
+    djph = cs * minmod4(Real{4} * d0s - dps, Real{4} * dps - d0s, d0s, dps);
+    djmh = cs * minmod4(Real{4} * d0s - dms, Real{4} * dms - d0s, d0s, dms);
+  }
+  const Real v_ul = v0 + alpha * d_minus;
+  const Real v_av = Real{0.5} * v0 + Real{0.5} * vp1;
+  const Real v_md = v_av - Real{0.5} * djph;
+  const Real v_lc = v0 + Real{0.5} * d_minus + (Real{4} / Real{3}) * djmh;
+  const Real vmin = std::fmax(std::fmin(std::fmin(v0, vp1), v_md),
+                              std::fmin(std::fmin(v0, v_ul), v_lc));
+  const Real vmax = std::fmin(std::fmax(std::fmax(v0, vp1), v_md),
+                              std::fmax(std::fmax(v0, v_ul), v_lc));
+  return median3(vl, vmin, vmax);
+}
+
+inline Real mp5_interp(Real a, Real b, Real c, Real d, Real e) {  // :182-191
+  const Real scale = std::fmax(std::fmax(std::fabs(a), std::fabs(b)),
+                               std::fmax(std::fabs(c), std::fmax(std::fabs(d), std::fabs(e))));
+  if (scale == Real{0}) return Real{0};
+  const Real sum = Real{2} * (a / scale) - Real{13} * (b / scale) +
+                   Real{47} * (c / scale) + Real{27} * (d / scale) -
+                   Real{3} * (e / scale);
+  return (sum / Real{60}) * scale;
+}
+
+inline Real mp7_interp(Real a, Real b, Real c, Real d, Real e, Real f, Real g) {  // :196-207
+  const Real scale = std::fmax(std::fmax(std::fmax(std::fabs(a), std::fabs(b)),
+                                         std::fmax(std::fabs(c), std::fabs(d))),
+                               std::fmax(std::fabs(e), std::fmax(std::fabs(f), std::fabs(g))));
+  if (scale == Real{0}) return Real{0};
+  const Real sum = -Real{3} * (a / scale) + Real{25} * (b / scale) -
+                   Real{101} * (c / scale) + Real{319} * (d / scale) +
+                   Real{214} * (e / scale) - Real{38} * (f / scale) +
+                   Real{4} * (g / scale);
+  return (sum / Real{420}) * scale;
+}
+
+inline Real mp5_reconstruct(const Real* v, int* clipped = nullptr) {  // :211-215
+  return mp_limit(mp5_interp(v[0], v[1], v[2], v[3], v[4]), v[0], v[1], v[2], v[3],
+                  v[4], clipped);
+}
+inline Real mp7_reconstruct(const Real* v, int* clipped = nullptr) {  // :219-224
+  return mp_limit(mp7_interp(v[0], v[1], v[2], v[3], v[4], v[5], v[6]), v[1], v[2],
+                  v[3], v[4], v[5], clipped);
+}
+
+// ---------------------------------------------------------------------------
+// Seven-wave eigensystem about a conserved state, normal +x
+// (include/quasar/numerics/mhd_eigensystem.hpp:58-431, zero background).
+struct Eigen {
+  Real L[49];  // rows = waves
+  Real R[49];  // R[var*7 + k], column k = wave
+  Real speeds[7];
+  bool identity_fallback{false};
+};
+
+inline void eigen_build(const State& s, Real gamma, Eigen& e) {
+  const Prim w = to_primitive(s, gamma);
+  const Real rho = w.rho;
+  const Real p = (w.p > Real{0}) ? w.p : Real{0};
+  const Real sqrt_rho = std::sqrt(rho);
+  // dir == 0 frame with a zero background: B components gain "+ 0".
+  const Real bn = w.bx + Real{0};
+  const Real bt1 = w.by + Real{0};
+  const Real bt2 = w.bz + Real{0};
+  const Real bpert_t1 = w.by, bpert_t2 = w.bz;
+  (void)bpert_t1;
+  (void)bpert_t2;
+  Real cf, cs, a, cax, ca;
+  magnetosonic_speeds(rho, p, bn, bt1, bt2, 0, gamma, cf, cs, a, cax, ca);
+
+  Real alpha_f, alpha_s;
+  {
+    const Real speed_scale = (cf > Real{0}) ? cf : Real{1};
+    const Real cfn = cf / speed_scale;
+    const Real csn = cs / speed_scale;
+    const Real an = a / speed_scale;
+    const Real denom = cfn * cfn - csn * csn;
+    constexpr Real eps = Real{64} * kEps;
+    if (denom <= eps * (cfn * cfn + csn * csn)) {
+      alpha_f = Real{1} / std::sqrt(Real{2});
+      alpha_s = Real{1} / std::sqrt(Real{2});
+    } else {
+      Real af2 = (an * an - csn * csn) / denom;
+      Real as2 = (cfn * cfn - an * an) / denom;
+      if (af2 < Real{0}) af2 = Real{0};
+      if (as2 < Real{0}) as2 = Real{0};
+      alpha_f = std::sqrt(af2);
+      alpha_s = std::sqrt(as2);
+    }
+  }
+  Real beta1, beta2;
+  {
+    const Real bt = scaled_norm3(bt1, bt2, Real{0});
+    const Real bt_speed = bt / sqrt_rho;
+    const Real field_scale = (a > ca) ? a : ca;
+    constexpr Real eps = Real{64} * kEps;
+    if (bt == Real{0} || bt_speed <= eps * (field_scale + kMin)) {
+      beta1 = Real{1} / std::sqrt(Real{2});
+      beta2 = Real{1} / std::sqrt(Real{2});
+    } else {
+      beta1 = bt1 / bt;
+      beta2 = bt2 / bt;
+    }
+  }
+  const Real sbn = (bn >= Real{0}) ? Real{1} : Real{-1};
+  const Real vn = w.vx, vt1 = w.vy, vt2 = w.vz;
+  e.speeds[0] = vn - cf;
+  e.speeds[1] = vn - cax;
+  e.speeds[2] = vn - cs;
+  e.speeds[3] = vn;
+  e.speeds[4] = vn + cs;
+  e.speeds[5] = vn + cax;
+  e.speeds[6] = vn + cf;
+
+  const Real qf = cf * alpha_f * sbn;
+  const Real qs = cs * alpha_s * sbn;
+  const Real Af = a * alpha_f * sqrt_rho;
+  const Real As = a * alpha_s * sqrt_rho;
+  const Real inv_a2 = (a > Real{0}) ? (Real{1} / a) / a : Real{0};
+  const Real Nf = Real{0.5} * inv_a2;
+  const Real rho_a2 = gamma * p;
+
+  Real Rp[7][7] = {{0}};
+  Rp[0][0] = rho * alpha_f;
+  Rp[0][1] = -alpha_f * cf;
+  Rp[0][2] = qs * beta1;
+  Rp[0][3] = qs * beta2;
+  Rp[0][4] = alpha_f * rho_a2;
+  Rp[0][5] = As * beta1;
+  Rp[0][6] = As * beta2;
+  Rp[6][0] = rho * alpha_f;
+  Rp[6][1] = alpha_f * cf;
+  Rp[6][2] = -qs * beta1;
+  Rp[6][3] = -qs * beta2;
+  Rp[6][4] = alpha_f * rho_a2;
+  Rp[6][5] = As * beta1;
+  Rp[6][6] = As * beta2;
+  {
+    const Real s2 = Real{1} / std::sqrt(Real{2});
+    Rp[1][2] = -beta2 * s2;
+    Rp[1][3] = beta1 * s2;
+    Rp[1][5] = -sbn * beta2 * s2 * sqrt_rho;
+    Rp[1][6] = sbn * beta1 * s2 * sqrt_rho;
+    Rp[5][2] = -beta2 * s2;
+    Rp[5][3] = beta1 * s2;
+    Rp[5][5] = sbn * beta2 * s2 * sqrt_rho;
+    Rp[5][6] = -sbn * beta1 * s2 * sqrt_rho;
+  }
+  Rp[2][0] = rho * alpha_s;
+  Rp[2][1] = -alpha_s * cs;
+  Rp[2][2] = -qf * beta1;
+  Rp[2][3] = -qf * beta2;
+  Rp[2][4] = alpha_s * rho_a2;
+  Rp[2][5] = -Af * beta1;
+  Rp[2][6] = -Af * beta2;
+  Rp[4][0] = rho * alpha_s;
+  Rp[4][1] = alpha_s * cs;
+  Rp[4][2] = qf * beta1;
+  Rp[4][3] = qf * beta2;
+  Rp[4][4] = alpha_s * rho_a2;
+  Rp[4][5] = -Af * beta1;
+  Rp[4][6] = -Af * beta2;
+  Rp[3][0] = Real{1};
+
+  Real Lp[7][7] = {{0}};
+  {
+    const Real s2 = Real{1} / std::sqrt(Real{2});
+    Lp[0][1] = -Nf * alpha_f * cf;
+    Lp[0][2] = Nf * qs * beta1;
+    Lp[0][3] = Nf * qs * beta2;
+    Lp[0][4] = Nf * alpha_f / rho;
+    Lp[0][5] = Nf * As * beta1 / rho;
+    Lp[0][6] = Nf * As * beta2 / rho;
+    Lp[6][1] = Nf * alpha_f * cf;
+    Lp[6][2] = -Nf * qs * beta1;
+    Lp[6][3] = -Nf * qs * beta2;
+    Lp[6][4] = Nf * alpha_f / rho;
+    Lp[6][5] = Nf * As * beta1 / rho;
+    Lp[6][6] = Nf * As * beta2 / rho;
+    Lp[1][2] = -beta2 * s2;
+    Lp[1][3] = beta1 * s2;
+    Lp[1][5] = -sbn * beta2 * s2 / sqrt_rho;
+    Lp[1][6] = sbn * beta1 * s2 / sqrt_rho;
+    Lp[5][2] = -beta2 * s2;
+    Lp[5][3] = beta1 * s2;
+    Lp[5][5] = sbn * beta2 * s2 / sqrt_rho;
+    Lp[5][6] = -sbn * beta1 * s2 / sqrt_rho;
+    Lp[2][1] = -Nf * alpha_s * cs;
+    Lp[2][2] = -Nf * qf * beta1;
+    Lp[2][3] = -Nf * qf * beta2;
+    Lp[2][4] = Nf * alpha_s / rho;
+    Lp[2][5] = -Nf * Af * beta1 / rho;
+    Lp[2][6] = -Nf * Af * beta2 / rho;
+    Lp[4][1] = Nf * alpha_s * cs;
+    Lp[4][2] = Nf * qf * beta1;
+    Lp[4][3] = Nf * qf * beta2;
+    Lp[4][4] = Nf * alpha_s / rho;
+    Lp[4][5] = -Nf * Af * beta1 / rho;
+    Lp[4][6] = -Nf * Af * beta2 / rho;
+    Lp[3][0] = Real{1};
+    Lp[3][4] = -inv_a2;
+  }
+
+  const Real gm1 = gamma - Real{1};
+  const Real half_v2 = half_squared_norm3(vn, vt1, vt2);
+  Real M[7][7] = {{0}};
+  M[0][0] = Real{1};
+  M[1][0] = vn;
+  M[1][1] = rho;
+  M[2][0] = vt1;
+  M[2][2] = rho;
+  M[3][0] = vt2;
+  M[3][3] = rho;
+  M[4][0] = half_v2;
+  M[4][1] = rho * vn;
+  M[4][2] = rho * vt1;
+  M[4][3] = rho * vt2;
+  M[4][4] = Real{1} / gm1;
+  M[4][5] = bt1;
+  M[4][6] = bt2;
+  M[5][5] = Real{1};
+  M[6][6] = Real{1};
+  Real Minv[7][7] = {{0}};
+  const Real inv_rho = Real{1} / rho;
+  Minv[0][0] = Real{1};
+  Minv[1][0] = -vn * inv_rho;
+  Minv[1][1] = inv_rho;
+  Minv[2][0] = -vt1 * inv_rho;
+  Minv[2][2] = inv_rho;
+  Minv[3][0] = -vt2 * inv_rho;
+  Minv[3][3] = inv_rho;
+  Minv[4][0] = gm1 * half_v2;
+  Minv[4][1] = -gm1 * vn;
+  Minv[4][2] = -gm1 * vt1;
+  Minv[4][3] = -gm1 * vt2;
+  Minv[4][4] = gm1;
+  Minv[4][5] = -gm1 * bt1;
+  Minv[4][6] = -gm1 * bt2;
+  Minv[5][5] = Real{1};
+  Minv[6][6] = Real{1};
+
+  for (int k = 0; k < 7; ++k) {
+    for (int var = 0; var < 7; ++var) {
+      Real racc = Real{0}, lacc = Real{0};
+      for (int j = 0; j < 7; ++j) {
+        racc += M[var][j] * Rp[k][j];
+        lacc += Lp[k][j] * Minv[j][var];
+      }
+      e.R[var * 7 + k] = racc;
+      e.L[k * 7 + var] = lacc;
+    }
+  }
+  bool usable = true;
+  for (int i = 0; i < 7 && usable; ++i) {
+    for (int j = 0; j < 7; ++j) {
+      Real dot = Real{0};
+      for (int k = 0; k < 7; ++k) dot += e.L[i * 7 + k] * e.R[k * 7 + j];
+      const Real expected = (i == j) ? Real{1} : Real{0};
+      if (!std::isfinite(dot) || std::fabs(dot - expected) > Real{1024} * kEps) {
+        usable = false;
+        break;
+      }
+    }
+  }
+  e.identity_fallback = !usable;
+  if (!usable) {
+    for (int i = 0; i < 7; ++i)
+      for (int j = 0; j < 7; ++j) {
+        e.L[i * 7 + j] = (i == j) ? Real{1} : Real{0};
+        e.R[i * 7 + j] = (i == j) ? Real{1} : Real{0};
+      }
+  }
+}
+
+// Characteristic projection in the +x frame
+// (include/quasar/numerics/characteristic_projection.hpp:36-83, dir == 0).
+inline void to_char(const State& d, const Eigen& e, Real* w) {
+  const Real u7[7] = {d.rho, d.mx, d.my, d.mz, d.energy, d.by, d.bz};
+  for (int k = 0; k < 7; ++k) {
+    Real acc = Real{0};
+    for (int var = 0; var < 7; ++var) acc += e.L[k * 7 + var] * u7[var];
+    w[k] = acc;
+  }
+}
+
+inline State from_char(const Real* w, const Eigen& e) {
+  Real u7[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < 7; ++k)
+    for (int var = 0; var < 7; ++var) u7[var] += e.R[var * 7 + k] * w[k];
+  State d;
+  d.rho = u7[0];
+  d.mx = u7[1];
+  d.my = u7[2];
+  d.mz = u7[3];
+  d.energy = u7[4];
+  d.by = u7[5];
+  d.bz = u7[6];
+  d.bx = Real{0};
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// HLLD / HLL / LF along +x (include/quasar/numerics/hlld_core.hpp:46-457).
+inline State rotate(const State& s, int dir) {  // hlld_core.hpp:46-61 (self-inverse)
+  if (dir == 0) return s;
+  State r = s;
+  r.mx = s.my;
+  r.my = s.mx;
+  r.bx = s.by;
+  r.by = s.bx;
+  return r;
+}
+
+inline Flux physical_flux_x(const State& u, Real gamma) {  // hlld_core.hpp:123-153
+  const Real vx = u.mx / u.rho;
+  const Real vy = u.my / u.rho;
+  const Real vz = u.mz / u.rho;
+  const Real p = pressure(u, gamma);
+  const Real half_bn2 = half_squared_norm3(u.bx, Real{0}, Real{0});
+  const Real half_bt2 = half_squared_norm3(Real{0}, u.by, u.bz);
+  Flux f;
+  f.rho = u.mx;
+  {
+    const Real a[5] = {u.mx, p, u.by, u.bz, u.bx};
+    const Real b[5] = {vx, Real{1}, Real{0.5} * u.by, Real{0.5} * u.bz, Real{-0.5} * u.bx};
+    f.mx = product_sum(a, b, 5);
+  }
+  f.my = product_sum2(u.mx, vy, u.bx, -u.by);
+  f.mz = product_sum2(u.mx, vz, u.bx, -u.bz);
+  const Real normal_enthalpy = ((u.energy - half_bn2) + p) + half_bt2;
+  const Real tangential_vdotb = product_sum2(vy, u.by, vz, u.bz);
+  f.energy = product_sum2(vx, normal_enthalpy, u.bx, -tangential_vdotb);
+  f.bx = Real{0};
+  f.by = product_sum2(u.by, vx, u.bx, -vy);
+  f.bz = product_sum2(u.bz, vx, u.bx, -vz);
+  return f;
+}
+
+inline Flux hll_flux_x(const State& L, const State& R, Real sl, Real sr,
+                       Real gamma) {  // hlld_core.hpp:157-192
+  const Flux fl = physical_flux_x(L, gamma);
+  const Flux fr = physical_flux_x(R, gamma);
+  if (sl >= Real{0}) return fl;
+  if (sr <= Real{0}) return fr;
+  const Real speed_scale = std::fmax(std::fabs(sl), std::fabs(sr));
+  if (!(speed_scale > Real{0}) || !std::isfinite(speed_scale)) return fl;
+  const Real sln = sl / speed_scale;
+  const Real srn = sr / speed_scale;
+  const Real denom = srn - sln;
+  if (!(denom > Real{0}) || !std::isfinite(denom)) return fl;
+  const Real wl = srn / denom;
+  const Real wr = -sln / denom;
+  const Real dissipation = speed_scale * (sln * srn / denom);
+  Flux f;
+  f.rho = product_sum3(wl, fl.rho, wr, fr.rho, dissipation, R.rho - L.rho);
+  f.mx = product_sum3(wl, fl.mx, wr, fr.mx, dissipation, R.mx - L.mx);
+  f.my = product_sum3(wl, fl.my, wr, fr.my, dissipation, R.my - L.my);
+  f.mz = product_sum3(wl, fl.mz, wr, fr.mz, dissipation, R.mz - L.mz);
+  f.energy = product_sum3(wl, fl.energy, wr, fr.energy, dissipation, R.energy - L.energy);
+  f.bx = Real{0};
+  f.by = product_sum3(wl, fl.by, wr, fr.by, dissipation, R.by - L.by);
+  f.bz = product_sum3(wl, fl.bz, wr, fr.bz, dissipation, R.bz - L.bz);
+  return f;
+}
+
+inline Flux lax_friedrichs_flux_x(const State& L, const State& R,
+                                  Real gamma) {  // hlld_core.hpp:200-210
+  const Real vxL = L.mx / L.rho;
+  const Real vxR = R.mx / R.rho;
+  const Real cfL = fast_speed(L, 0, gamma);
+  const Real cfR = fast_speed(R, 0, gamma);
+  const Real alphaL = std::fabs(vxL) + cfL;
+  const Real alphaR = std::fabs(vxR) + cfR;
+  const Real alpha = (alphaL > alphaR) ? alphaL : alphaR;
+  return hll_flux_x(L, R, -alpha, alpha, gamma);
+}
+
+// Branch taken by hlld_flux_x, reported for label generation.
+enum HlldPath { kPhysical = 0, kHllFallback = 1, kStar = 2, kDoubleStar = 3 };
+
+inline Flux hlld_flux_x(const State& Lin, const State& Rin, Real gamma,
+                        int* path = nullptr) {  // hlld_core.hpp:213-457
+  constexpr Real rel_tol = Real{64} * kEps;
+  auto resolved = [&](Real value, Real scale) {
+    return std::isfinite(value) && std::fabs(value) > rel_tol * (scale + kMin);
+  };
+  auto hll_with = [&](const State& L, const State& R, Real sl, Real sr) {
+    if (path) *path = kHllFallback;
+    return hll_flux_x(L, R, sl, sr, gamma);
+  };
+  if (Lin.rho == Rin.rho && Lin.mx == Rin.mx && Lin.my == Rin.my &&
+      Lin.mz == Rin.mz && Lin.energy == Rin.energy && Lin.bx == Rin.bx &&
+      Lin.by == Rin.by && Lin.bz == Rin.bz) {
+    if (path) *path = kPhysical;
+    return physical_flux_x(Lin, gamma);
+  }
+  const Real bn = Real{0.5} * Lin.bx + Real{0.5} * Rin.bx;
+  State L = Lin, R = Rin;
+  const Real half_bn2 = half_squared_norm3(bn, Real{0}, Real{0});
+  L.energy = (L.energy - half_squared_norm3(L.bx, Real{0}, Real{0})) + half_bn2;
+  R.energy = (R.energy - half_squared_norm3(R.bx, Real{0}, Real{0})) + half_bn2;
+  L.bx = bn;
+  R.bx = bn;
+  const bool bn2_representable =
+      half_bn2 <= Real{0.5} * std::numeric_limits<Real>::max();
+  const Real bn2 = bn2_representable ? Real{2} * half_bn2 : Real{0};
+
+  const Real inv_rhoL = Real{1} / L.rho;
+  const Real inv_rhoR = Real{1} / R.rho;
+  const Real vxL = L.mx * inv_rhoL, vyL = L.my * inv_rhoL, vzL = L.mz * inv_rhoL;
+  const Real vxR = R.mx * inv_rhoR, vyR = R.my * inv_rhoR, vzR = R.mz * inv_rhoR;
+  const Real pL = pressure(L, gamma);
+  const Real pR = pressure(R, gamma);
+  const Real ptL = pL + half_squared_norm3(L.bx, L.by, L.bz);
+  const Real ptR = pR + half_squared_norm3(R.bx, R.by, R.bz);
+  const Real cfL = fast_speed(L, 0, gamma);
+  const Real cfR = fast_speed(R, 0, gamma);
+  const Real cfmax = (cfL > cfR) ? cfL : cfR;
+  const Real sl = ((vxL < vxR) ? vxL : vxR) - cfmax;
+  const Real sr = ((vxL > vxR) ? vxL : vxR) + cfmax;
+  if (sl >= Real{0}) {
+    if (path) *path = kPhysical;
+    return physical_flux_x(L, gamma);
+  }
+  if (sr <= Real{0}) {
+    if (path) *path = kPhysical;
+    return physical_flux_x(R, gamma);
+  }
+  if (!bn2_representable || !std::isfinite(ptL) || !std::isfinite(ptR))
+    return hll_with(L, R, sl, sr);
+
+  const Real denomM = (sr - vxR) * R.rho - (sl - vxL) * L.rho;
+  const Real denomM_scale = std::fabs((sr - vxR) * R.rho) + std::fabs((sl - vxL) * L.rho);
+  if (!resolved(denomM, denomM_scale)) return hll_with(L, R, sl, sr);
+  const Real sm =
+      ((sr - vxR) * R.rho * vxR - (sl - vxL) * L.rho * vxL - ptR + ptL) / denomM;
+  if (!std::isfinite(sm)) return hll_with(L, R, sl, sr);
+  const Real pt_star = ((sr - vxR) * R.rho * ptL - (sl - vxL) * L.rho * ptR +
+                        L.rho * R.rho * (sr - vxR) * (sl - vxL) * (vxR - vxL)) /
+                       denomM;
+  if (!std::isfinite(pt_star)) return hll_with(L, R, sl, sr);
+  const Real dLs = sl - sm;
+  const Real dRs = sr - sm;
+  const Real wave_scale = std::fabs(sl) + std::fabs(sr) + std::fabs(sm);
+  if (!resolved(dLs, wave_scale) || !resolved(dRs, wave_scale))
+    return hll_with(L, R, sl, sr);
+  const Real rhoLs = L.rho * (sl - vxL) / dLs;
+  const Real rhoRs = R.rho * (sr - vxR) / dRs;
+  if (!(rhoLs > Real{0}) || !(rhoRs > Real{0})) return hll_with(L, R, sl, sr);
+
+  struct Star {
+    State u;
+    Real vy, vz, by, bz;
+    bool ok;
+  };
+  auto star_state = [&](const State& U, Real s, Real vx, Real vy, Real vz, Real pt,
+                        Real rhos) -> Star {
+    Star out{};
+    out.ok = true;
+    const Real sMinusV = s - vx;
+    const Real denom = U.rho * sMinusV * (s - sm) - bn2;
+    const Real denom_scale = std::fabs(U.rho * sMinusV * (s - sm)) + bn2;
+    State us;
+    us.rho = rhos;
+    us.bx = bn;
+    if (!resolved(denom, denom_scale)) {
+      out.ok = false;
+      return out;
+    }
+    const Real inv_denom = Real{1} / denom;
+    const Real fac = bn * (sm - vx) * inv_denom;
+    const Real vys = vy - U.by * fac;
+    const Real vzs = vz - U.bz * fac;
+    const Real bcoef = (U.rho * sMinusV * sMinusV - bn2) * inv_denom;
+    const Real bys = U.by * bcoef;
+    const Real bzs = U.bz * bcoef;
+    us.mx = us.rho * sm;
+    us.my = us.rho * vys;
+    us.mz = us.rho * vzs;
+    us.by = bys;
+    us.bz = bzs;
+    const Real vdotb = product_sum3(vx, U.bx, vy, U.by, vz, U.bz);
+    const Real vdotb_s = product_sum3(sm, bn, vys, bys, vzs, bzs);
+    us.energy = (sMinusV * U.energy - pt * vx + pt_star * sm + bn * (vdotb - vdotb_s)) /
+                (s - sm);
+    out.u = us;
+    out.vy = vys;
+    out.vz = vzs;
+    out.by = bys;
+    out.bz = bzs;
+    return out;
+  };
+  const Star sL = star_state(L, sl, vxL, vyL, vzL, ptL, rhoLs);
+  const Star sR = star_state(R, sr, vxR, vyR, vzR, ptR, rhoRs);
+  if (!sL.ok || !sR.ok) return hll_with(L, R, sl, sr);
+  const State usL = sL.u, usR = sR.u;
+  const Flux fl = physical_flux_x(L, gamma);
+  const Flux fr = physical_flux_x(R, gamma);
+  auto flux_from_star = [&](const Flux& fU, const State& U, const State& us, Real s) {
+    Flux f;
+    f.rho = fU.rho + s * (us.rho - U.rho);
+    f.mx = fU.mx + s * (us.mx - U.mx);
+    f.my = fU.my + s * (us.my - U.my);
+    f.mz = fU.mz + s * (us.mz - U.mz);
+    f.energy = fU.energy + s * (us.energy - U.energy);
+    f.bx = Real{0};
+    f.by = fU.by + s * (us.by - U.by);
+    f.bz = fU.bz + s * (us.bz - U.bz);
+    return f;
+  };
+  const Real sqrt_rhoLs = std::sqrt(rhoLs);
+  const Real sqrt_rhoRs = std::sqrt(rhoRs);
+  const Real saL = sm - std::fabs(bn) / sqrt_rhoLs;
+  const Real saR = sm + std::fabs(bn) / sqrt_rhoRs;
+  const bool bn_negligible =
+      bn2 <= rel_tol * (std::fabs(pt_star) + std::fabs(ptL) + std::fabs(ptR) + kMin);
+  Flux f;
+  if (bn_negligible) {
+    if (path) *path = kStar;
+    if (sl <= Real{0} && Real{0} <= sm) {
+      f = flux_from_star(fl, L, usL, sl);
+    } else {
+      f = flux_from_star(fr, R, usR, sr);
+    }
+    if (!std::isfinite(f.rho) || !std::isfinite(f.energy)) return hll_with(L, R, sl, sr);
+    return f;
+  }
+  const Real inv_sqrt_sum = Real{1} / (sqrt_rhoLs + sqrt_rhoRs);
+  const Real sgn_bn = (bn >= Real{0}) ? Real{1} : Real{-1};
+  const Real vyss =
+      (sqrt_rhoLs * sL.vy + sqrt_rhoRs * sR.vy + (sR.by - sL.by) * sgn_bn) * inv_sqrt_sum;
+  const Real vzss =
+      (sqrt_rhoLs * sL.vz + sqrt_rhoRs * sR.vz + (sR.bz - sL.bz) * sgn_bn) * inv_sqrt_sum;
+  const Real byss = (sqrt_rhoLs * sR.by + sqrt_rhoRs * sL.by +
+                     sqrt_rhoLs * sqrt_rhoRs * (sR.vy - sL.vy) * sgn_bn) *
+                    inv_sqrt_sum;
+  const Real bzss = (sqrt_rhoLs * sR.bz + sqrt_rhoRs * sL.bz +
+                     sqrt_rhoLs * sqrt_rhoRs * (sR.vz - sL.vz) * sgn_bn) *
+                    inv_sqrt_sum;
+  auto double_star = [&](const Star& s, Real sqrt_rhos, Real side_sign) {
+    State uss;
+    uss.rho = s.u.rho;
+    uss.bx = bn;
+    uss.mx = uss.rho * sm;
+    uss.my = uss.rho * vyss;
+    uss.mz = uss.rho * vzss;
+    uss.by = byss;
+    uss.bz = bzss;
+    const Real vdotb_ss = product_sum3(sm, bn, vyss, byss, vzss, bzss);
+    const Real vdotb_s = product_sum3(sm, bn, s.vy, s.by, s.vz, s.bz);
+    uss.energy = s.u.energy + side_sign * sqrt_rhos * sgn_bn * (vdotb_s - vdotb_ss);
+    return uss;
+  };
+  const State ussL = double_star(sL, sqrt_rhoLs, Real{-1});
+  const State ussR = double_star(sR, sqrt_rhoRs, Real{+1});
+  if (path) *path = kDoubleStar;
+  if (Real{0} <= saL) {
+    f = flux_from_star(fl, L, usL, sl);
+  } else if (Real{0} <= sm) {
+    f.rho = fl.rho + saL * (ussL.rho - usL.rho) + sl * (usL.rho - L.rho);
+    f.mx = fl.mx + saL * (ussL.mx - usL.mx) + sl * (usL.mx - L.mx);
+    f.my = fl.my + saL * (ussL.my - usL.my) + sl * (usL.my - L.my);
+    f.mz = fl.mz + saL * (ussL.mz - usL.mz) + sl * (usL.mz - L.mz);
+    f.energy = fl.energy + saL * (ussL.energy - usL.energy) + sl * (usL.energy - L.energy);
+    f.bx = Real{0};
+    f.by = fl.by + saL * (ussL.by - usL.by) + sl * (usL.by - L.by);
+    f.bz = fl.bz + saL * (ussL.bz - usL.bz) + sl * (usL.bz - L.bz);
+  } else if (Real{0} <= saR) {
+    f.rho = fr.rho + saR * (ussR.rho - usR.rho) + sr * (usR.rho - R.rho);
+    f.mx = fr.mx + saR * (ussR.mx - usR.mx) + sr * (usR.mx - R.mx);
+    f.my = fr.my + saR * (ussR.my - usR.my) + sr * (usR.my - R.my);
+    f.mz = fr.mz + saR * (ussR.mz - usR.mz) + sr * (usR.mz - R.mz);
+    f.energy = fr.energy + saR * (ussR.energy - usR.energy) + sr * (usR.energy - R.energy);
+    f.bx = Real{0};
+    f.by = fr.by + saR * (ussR.by - usR.by) + sr * (usR.by - R.by);
+    f.bz = fr.bz + saR * (ussR.bz - usR.bz) + sr * (usR.bz - R.bz);
+  } else {
+    f = flux_from_star(fr, R, usR, sr);
+  }
+  if (!std::isfinite(f.rho) || !std::isfinite(f.mx) || !std::isfinite(f.my) ||
+      !std::isfinite(f.mz) || !std::isfinite(f.energy) || !std::isfinite(f.by) ||
+      !std::isfinite(f.bz))
+    return hll_with(L, R, sl, sr);
+  return f;
+}
+
+}  // namespace oracle

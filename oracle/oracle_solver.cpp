@@ -1,0 +1,601 @@
+// ORACLE — test infrastructure only.  Restatement of the MhdSolver2D step driver
+// (src/physics/mhd/mhd_solver.cpp) and the C ABI used by tests and by the
+// CPU-baseline leg of bench.py.
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "oracle_capi.h"
+#include "oracle_grid.hpp"
+
+namespace oracle {
+namespace {
+
+thread_local std::string g_last_error;
+
+Grid to_grid(const OrcGrid* g) {
+  Grid out;
+  out.nx = g->nx;
+  out.ny = g->ny;
+  out.ng = g->ng;
+  out.lx = g->lx;
+  out.ly = g->ly;
+  out.ox = g->ox;
+  out.oy = g->oy;
+  out.cell_dx = g->cell_dx;
+  out.cell_dy = g->cell_dy;
+  return out;
+}
+
+Field to_field(double* const* p) {
+  Field f;
+  for (int c = 0; c < 8; ++c) f.c[c] = p[c];
+  return f;
+}
+Iface to_iface(double* const* p) {
+  Iface f;
+  for (int c = 0; c < 16; ++c) f.c[c] = p[c];
+  return f;
+}
+Flags to_flags(const int* f) {
+  Flags out;
+  for (int s = 0; s < 4; ++s) out.side[s] = f ? f[s] : 0;
+  return out;
+}
+State to_state(const double* u) {
+  return State{u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7]};
+}
+void from_state(const State& s, double* u) {
+  u[0] = s.rho;
+  u[1] = s.mx;
+  u[2] = s.my;
+  u[3] = s.mz;
+  u[4] = s.energy;
+  u[5] = s.bx;
+  u[6] = s.by;
+  u[7] = s.bz;
+}
+
+struct PositivityRetry {
+  Real theta;
+};
+
+// Owned padded planes.
+struct Planes {
+  std::vector<Real> data[8];
+  void resize(std::size_t n) {
+    for (auto& d : data) d.assign(n, Real{0});
+  }
+  Field field() {
+    Field f;
+    for (int c = 0; c < 8; ++c) f.c[c] = data[c].data();
+    return f;
+  }
+};
+
+class Solver {
+ public:
+  Solver(const Grid& g, Real gamma, Real cfl, int order, const Flags& fluid, const Flags& field)
+      : g_(g), gamma_(gamma), cfl_(cfl), order_(order), fluid_(fluid), field_(field) {
+    const std::size_t n = g_.size();
+    for (auto& r : rk_) r.resize(n);
+    step_backup_.resize(n);
+    request_backup_.resize(n);
+    residual_.resize(n);
+    flux_x_.resize(n);
+    flux_y_.resize(n);
+    for (auto& p : ifx_) p.assign(n, Real{0});
+    for (auto& p : ify_) p.assign(n, Real{0});
+    cell_ez_.assign(n, 0);
+    xface_ez_.assign(n, 0);
+    yface_ez_.assign(n, 0);
+    ez_edge_.assign(n, 0);
+    xnj_.assign(n, 0);
+    ynj_.assign(n, 0);
+  }
+
+  Grid grid() const { return g_; }
+  Planes& live() { return rk_[0]; }
+  Counters counters{};
+  int last_substeps{0};
+
+  void seed(int comp, const Real* data) {
+    for (std::size_t k = 0; k < g_.size(); ++k) {
+      if (!std::isfinite(data[k]))
+        throw std::invalid_argument("seed_state: host buffer must contain only finite values");
+    }
+    std::memcpy(rk_[0].data[comp].data(), data, g_.size() * sizeof(Real));
+    solver_owned_ = false;
+  }
+
+  Real cfl_limit(int co = 0) {  // mhd_solver.cpp:1228-1321
+    Field u = rk_[0].field();
+    fill_ghosts(g_, fluid_, field_, u);
+    ensure_solenoidal();
+    ensure_admissible(co);
+    const int order = co > 0 ? 1 : order_;
+    Iface fx = iface(ifx_), fy = iface(ify_);
+    reconstruct(g_, 0, order, field_, gamma_, u, fx, &counters);
+    reconstruct(g_, 1, order, field_, gamma_, u, fy, &counters);
+    const Real max_rate = cfl_max_rate(g_, order, field_, gamma_, fx, fy);
+    const Real effective = order >= 5 ? std::min(cfl_, Real{0.2}) : cfl_;
+    if (!std::isfinite(max_rate))
+      throw std::runtime_error("MhdSolver2D: non-finite state encountered in CFL reduction");
+    if (!(max_rate > Real{0})) {
+      const Real h_min = std::min(g_.dx(), g_.dy());
+      const Real h_max = std::max(g_.dx(), g_.dy());
+      return effective * (h_min / (Real{1} + h_min / h_max));
+    }
+    const Real result = effective / max_rate;
+    if (!(result > Real{0}) || !std::isfinite(result))
+      throw std::runtime_error("MhdSolver2D: no finite positive CFL timestep is representable");
+    return result;
+  }
+
+  void step_unchecked(Real dt) {  // mhd_solver.cpp:1399-1409
+    if (!(dt > Real{0}) || !std::isfinite(dt))
+      throw std::invalid_argument("MhdSolver2D: dt must be finite and positive");
+    ensure_admissible(0);
+    advance_positive(dt);
+  }
+
+  void step(Real dt) {  // mhd_solver.cpp:1384-1397
+    if (!(dt > Real{0}) || !std::isfinite(dt))
+      throw std::invalid_argument("MhdSolver2D: dt must be finite and positive");
+    if (dt > cfl_limit(0))
+      throw std::invalid_argument(
+          "MhdSolver2D: dt exceeds the CFL stability limit for this grid and scheme");
+    advance_positive(dt);
+  }
+
+  Real divergence_b_max() {
+    Field u = rk_[0].field();
+    fill_ghosts(g_, fluid_, field_, u);
+    return divb_linf(g_, u);
+  }
+
+  void totals(Real* mass, Real* energy) {
+    Field u = rk_[0].field();
+    conserved_totals(g_, u, mass, energy);
+  }
+
+  void collocated(int axis, Real* out) {
+    Field u = rk_[0].field();
+    for (int j = -g_.ng; j < g_.ny + g_.ng; ++j)
+      for (int i = -g_.ng; i < g_.nx + g_.ng; ++i)
+        out[g_.index(i, j)] = axis == 0 ? cell_bx(g_, u.c[5], i, j, 0) : cell_by(g_, u.c[6], i, j, 0);
+  }
+
+  void compute_residual_public(Field& dudt) {
+    Field u = rk_[0].field();
+    compute_residual_into(u, dudt, order_);
+  }
+
+ private:
+  static Iface iface(std::vector<Real> (&p)[16]) {
+    Iface f;
+    for (int c = 0; c < 16; ++c) f.c[c] = p[c].data();
+    return f;
+  }
+
+  void ensure_admissible(int co) {  // mhd_solver.cpp:1178-1194
+    Field u = rk_[0].field();
+    const Real theta = admissible_fraction(g_, u, u, Real{0}, Real{0}, co);
+    if (!(theta >= Real{1}))
+      throw std::invalid_argument(
+          "MhdSolver2D: live state must have finite components, rho > 0, and gas pressure > 0 "
+          "in every interior cell");
+  }
+
+  void ensure_solenoidal() {  // mhd_solver.cpp:1196-1222
+    Field u = rk_[0].field();
+    const Real rel = divb_relative_linf(g_, u, solver_owned_);
+    if (!(std::isfinite(rel) && rel <= Real{1024} * kEps))
+      throw std::invalid_argument("MhdSolver2D: live magnetic field is not discretely divergence-free");
+  }
+
+  void compute_residual_into(Field& u, Field& dudt, int order) {  // mhd_solver.cpp:665-826
+    const int co = order <= 1 ? 1 : 0;
+    fill_ghosts(g_, fluid_, field_, u);
+    if (u.c[0] == rk_[0].data[0].data()) {
+      ensure_solenoidal();
+      ensure_admissible(co);
+    }
+    for (int c = 0; c < 8; ++c) std::fill(dudt.c[c], dudt.c[c] + g_.size(), Real{0});
+    const int hll_only = order <= 1 ? 1 : 0;
+    Iface fx = iface(ifx_), fy = iface(ify_);
+    Field flx = flux_x_.field(), fly = flux_y_.field();
+    reconstruct(g_, 0, order, field_, gamma_, u, fx, &counters);
+    face_flux(g_, 0, order, hll_only, field_, gamma_, fx, flx, &counters);
+    reconstruct(g_, 1, order, field_, gamma_, u, fy, &counters);
+    face_flux(g_, 1, order, hll_only, field_, gamma_, fy, fly, &counters);
+    flux_difference(g_, 0, flx, dudt);
+    flux_difference(g_, 1, fly, dudt);
+    ct_emf(g_, order, hll_only, field_, gamma_, u, fx, fy, cell_ez_.data(), xface_ez_.data(),
+           yface_ez_.data(), xnj_.data(), ynj_.data(), ez_edge_.data());
+    emf_curl_rate(g_, ez_edge_.data(), dudt);
+  }
+
+  void copy_state(Planes& src, Planes& dst) {  // mhd_solver.cpp:955-963
+    Field s = src.field(), d = dst.field(), r = residual_.field();
+    rk_stage(g_, Real{1}, Real{0}, Real{0}, s, s, r, d);
+    fill_ghosts(g_, fluid_, field_, d);
+  }
+
+  Real stage_theta(int stage) {  // mhd_solver.cpp:908-953
+    Field out = rk_[stage == 2 ? 0 : stage + 1].field();
+    Field base = step_backup_.field();
+    Real theta = Real{1};
+    if (positivity_active_) {
+      theta = admissible_fraction(g_, base, out, Real{0}, Real{0}, positivity_order_);
+      if (stage == 2 && positivity_order_ == 0 && low_anchor_) {
+        theta = std::min(theta, admissible_fraction(g_, base, out, Real{0}, Real{0}, 1));
+      } else if (stage == 2 && positivity_order_ == 1) {
+        theta = std::min(theta, admissible_fraction(g_, base, out, Real{0}, Real{0}, 0));
+      }
+    }
+    return std::isfinite(theta) ? theta : Real{0};
+  }
+
+  void apply_stage(int stage, Real dt) {  // mhd_solver.cpp:840-906
+    Real a, b, c;
+    Planes* out;
+    switch (stage) {
+      case 0: out = &rk_[1]; a = Real{1}; b = Real{0}; c = dt; break;
+      case 1: out = &rk_[2]; a = Real{3} / Real{4}; b = Real{1} / Real{4}; c = (Real{1} / Real{4}) * dt; break;
+      default: out = &rk_[0]; a = Real{1} / Real{3}; b = Real{2} / Real{3}; c = (Real{2} / Real{3}) * dt; break;
+    }
+    Field un = rk_[0].field();
+    Field us = (stage == 2) ? rk_[2].field() : rk_[1].field();
+    Field o = out->field();
+    Field r = residual_.field();
+    rk_stage(g_, a, b, c, un, us, r, o);
+    fill_ghosts(g_, fluid_, field_, o);
+  }
+
+  void integrate(Real dt) {  // src/numerics/ssprk_integrator.cpp:40-53
+    const int order = positivity_order_ > 0 ? positivity_order_ : order_;
+    for (int s = 0; s < 3; ++s) {
+      Field u = rk_[s].field();
+      Field r = residual_.field();
+      compute_residual_into(u, r, order);
+      apply_stage(s, dt);
+      const Real theta = stage_theta(s);
+      if (!(theta >= Real{1})) throw PositivityRetry{std::isfinite(theta) ? theta : Real{0}};
+    }
+  }
+
+  void advance_positive(Real dt) {  // mhd_solver.cpp:965-1143
+    const int previous_substeps = last_substeps;
+    const bool previous_owned = solver_owned_;
+    copy_state(rk_[0], request_backup_);
+    try {
+      Real remaining = dt, trial = dt;
+      bool low_order_interval = false;
+      int retries = 0, attempts = 0;
+      last_substeps = 0;
+      {
+        Field u = rk_[0].field();
+        low_anchor_ = admissible_fraction(g_, u, u, Real{0}, Real{0}, 1) >= Real{1};
+      }
+      while (remaining > Real{0}) {
+        if (++attempts > 100000)
+          throw std::runtime_error("MhdSolver2D: positivity controller exceeded its substep limit");
+        trial = std::min(trial, remaining);
+        copy_state(rk_[0], step_backup_);
+        positivity_active_ = true;
+        try {
+          integrate(trial);
+        } catch (const PositivityRetry& retry) {
+          const bool high_order_attempt = positivity_order_ == 0;
+          positivity_active_ = false;
+          positivity_order_ = 0;
+          copy_state(step_backup_, rk_[0]);
+          if (++retries > 80)
+            throw std::runtime_error("MhdSolver2D: unable to find a positive conservative substep");
+          if (high_order_attempt) {
+            Field u = rk_[0].field();
+            const Real low_base = admissible_fraction(g_, u, u, Real{0}, Real{0}, 1);
+            if (low_base >= Real{1}) {
+              low_anchor_ = true;
+              low_order_interval = true;
+              positivity_order_ = 1;
+              trial = std::min(trial, cfl_limit(1));
+              if (!(trial > Real{0}) || !std::isfinite(trial) || !(remaining - trial < remaining)) {
+                positivity_order_ = 0;
+                throw std::runtime_error("MhdSolver2D: low-order CFL substep cannot advance time");
+              }
+              continue;
+            }
+          }
+          const Real suggested = Real{0.8} * retry.theta;
+          const Real factor = std::max(Real{0.05}, std::min(Real{0.8}, suggested));
+          trial *= factor;
+          positivity_order_ = low_order_interval ? 1 : 0;
+          if (!(trial > Real{0}) || !std::isfinite(trial) || !(remaining - trial < remaining)) {
+            positivity_order_ = 0;
+            throw std::runtime_error("MhdSolver2D: positive conservative substep cannot advance time");
+          }
+          continue;
+        } catch (...) {
+          positivity_active_ = false;
+          positivity_order_ = 0;
+          copy_state(step_backup_, rk_[0]);
+          throw;
+        }
+        positivity_active_ = false;
+        solver_owned_ = true;
+        retries = 0;
+        ++last_substeps;
+        positivity_order_ = low_order_interval ? 1 : 0;
+        if (trial == remaining) {
+          remaining = Real{0};
+          break;
+        }
+        const Real next = remaining - trial;
+        if (!(next > Real{0}) || !(next < remaining))
+          throw std::runtime_error("MhdSolver2D: positivity substep made no finite time progress");
+        remaining = next;
+        trial = std::min(std::min(Real{2} * trial, remaining), cfl_limit(low_order_interval ? 1 : 0));
+      }
+      positivity_active_ = false;
+      low_anchor_ = false;
+      positivity_order_ = 0;
+    } catch (...) {
+      positivity_active_ = false;
+      low_anchor_ = false;
+      positivity_order_ = 0;
+      last_substeps = previous_substeps;
+      copy_state(request_backup_, rk_[0]);
+      solver_owned_ = previous_owned;
+      throw;
+    }
+  }
+
+  Grid g_;
+  Real gamma_, cfl_;
+  int order_;
+  Flags fluid_, field_;
+  Planes rk_[3], step_backup_, request_backup_, residual_, flux_x_, flux_y_;
+  std::vector<Real> ifx_[16], ify_[16];
+  std::vector<Real> cell_ez_, xface_ez_, yface_ez_, ez_edge_;
+  std::vector<int> xnj_, ynj_;
+  bool solver_owned_{false};
+  bool positivity_active_{false};
+  bool low_anchor_{false};
+  int positivity_order_{0};
+};
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return 1;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 2;
+  }
+}
+
+void copy_counters(const Counters& c, OrcCounters* out) {
+  if (!out) return;
+  out->mp_clips += c.mp_clips;
+  out->muscl_fallbacks += c.muscl_fallbacks;
+  out->pc_fallbacks += c.pc_fallbacks;
+  out->sentinels += c.sentinels;
+  out->point_fallbacks += c.point_fallbacks;
+  out->hll_fallbacks += c.hll_fallbacks;
+}
+
+}  // namespace
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+double orc_cr_sum(const double* terms, int n) { return cr_sum(terms, n); }
+double orc_product_sum(const double* a, const double* b, int n) { return product_sum(a, b, n); }
+double orc_pressure(const double* u, double gamma) { return pressure(to_state(u), gamma); }
+void orc_physical_flux_x(const double* u, double gamma, double* f) {
+  from_state(physical_flux_x(to_state(u), gamma), f);
+}
+void orc_hll_flux_x(const double* l, const double* r, double sl, double sr, double gamma, double* f) {
+  from_state(hll_flux_x(to_state(l), to_state(r), sl, sr, gamma), f);
+}
+void orc_lf_flux_x(const double* l, const double* r, double gamma, double* f) {
+  from_state(lax_friedrichs_flux_x(to_state(l), to_state(r), gamma), f);
+}
+int orc_hlld_flux_x(const double* l, const double* r, double gamma, double* f) {
+  int path = -1;
+  from_state(hlld_flux_x(to_state(l), to_state(r), gamma, &path), f);
+  return path;
+}
+double orc_fast_speed(const double* u, int dir, double gamma) {
+  return fast_speed(to_state(u), dir, gamma);
+}
+double orc_mp_limit(double vl, const double* v) { return mp_limit(vl, v[0], v[1], v[2], v[3], v[4]); }
+double orc_mp5_interp(const double* v) { return mp5_interp(v[0], v[1], v[2], v[3], v[4]); }
+double orc_mp7_interp(const double* v) { return mp7_interp(v[0], v[1], v[2], v[3], v[4], v[5], v[6]); }
+double orc_mp5_reconstruct(const double* v) { return mp5_reconstruct(v); }
+double orc_mp7_reconstruct(const double* v) { return mp7_reconstruct(v); }
+int orc_eigen_build(const double* u, double gamma, double* L, double* R, double* speeds) {
+  Eigen e;
+  eigen_build(to_state(u), gamma, e);
+  std::memcpy(L, e.L, sizeof(e.L));
+  std::memcpy(R, e.R, sizeof(e.R));
+  std::memcpy(speeds, e.speeds, sizeof(e.speeds));
+  return e.identity_fallback ? 1 : 0;
+}
+void orc_to_char(const double* d8, const double* L49, double* w7) {
+  Eigen e{};
+  std::memcpy(e.L, L49, sizeof(e.L));
+  to_char(to_state(d8), e, w7);
+}
+double orc_point_value(int order, int node, const double* samples) {
+  return transverse_point_value(order, node, samples);
+}
+double orc_gauss_average(int order, const double* samples) {
+  return transverse_gauss_average(order, samples);
+}
+double orc_philox_uniform_pm1(uint64_t seed, uint32_t component, uint64_t cell) {
+  return philox_uniform_pm1(seed, component, cell);
+}
+void orc_philox4x32_10(uint32_t* ctr4, const uint32_t* key2) { philox4x32_10(ctr4, key2); }
+
+double orc_cell_bx(const OrcGrid* g, const double* f, int i, int j, int co) {
+  return cell_bx(to_grid(g), f, i, j, co);
+}
+double orc_cell_by(const OrcGrid* g, const double* f, int i, int j, int co) {
+  return cell_by(to_grid(g), f, i, j, co);
+}
+void orc_reconstruct(const OrcGrid* g, int dir, int order, const int* flags, double gamma,
+                     double* const* u8, double* const* out16, OrcCounters* counters) {
+  Counters c;
+  Iface out = to_iface(out16);
+  reconstruct(to_grid(g), dir, order, to_flags(flags), gamma, to_field(u8), out, &c);
+  copy_counters(c, counters);
+}
+void orc_face_flux(const OrcGrid* g, int dir, int order, int hll_only, const int* flags,
+                   double gamma, double* const* in16, double* const* flux8, OrcCounters* counters) {
+  Counters c;
+  Field f = to_field(flux8);
+  face_flux(to_grid(g), dir, order, hll_only, to_flags(flags), gamma, to_iface(in16), f, &c);
+  copy_counters(c, counters);
+}
+void orc_flux_difference(const OrcGrid* g, int dir, double* const* flux8, double* const* dudt8) {
+  Field d = to_field(dudt8);
+  flux_difference(to_grid(g), dir, to_field(flux8), d);
+}
+void orc_ct_emf(const OrcGrid* g, int order, int hll_only, const int* flags, double gamma,
+                double* const* u8, double* const* ifx16, double* const* ify16, double* cell_ez,
+                double* xface_ez, double* yface_ez, int* xnj, int* ynj, double* ez_edge) {
+  ct_emf(to_grid(g), order, hll_only, to_flags(flags), gamma, to_field(u8), to_iface(ifx16),
+         to_iface(ify16), cell_ez, xface_ez, yface_ez, xnj, ynj, ez_edge);
+}
+void orc_emf_curl_rate(const OrcGrid* g, const double* ez, double* const* dudt8) {
+  Field d = to_field(dudt8);
+  emf_curl_rate(to_grid(g), ez, d);
+}
+void orc_rk_stage(const OrcGrid* g, double a, double b, double c, double* const* un8,
+                  double* const* us8, double* const* dudt8, double* const* out8) {
+  Field o = to_field(out8);
+  rk_stage(to_grid(g), a, b, c, to_field(un8), to_field(us8), to_field(dudt8), o);
+}
+void orc_fill_ghosts(const OrcGrid* g, const int* fluid_flags, const int* field_flags,
+                     double* const* u8) {
+  Field u = to_field(u8);
+  fill_ghosts(to_grid(g), to_flags(fluid_flags), to_flags(field_flags), u);
+}
+double orc_cfl_max_rate(const OrcGrid* g, int order, const int* flags, double gamma,
+                        double* const* ifx16, double* const* ify16) {
+  return cfl_max_rate(to_grid(g), order, to_flags(flags), gamma, to_iface(ifx16), to_iface(ify16));
+}
+double orc_admissible_fraction(const OrcGrid* g, double* const* base8, double* const* cand8,
+                               double rho_floor, double q_floor, int co) {
+  return admissible_fraction(to_grid(g), to_field(base8), to_field(cand8), rho_floor, q_floor, co);
+}
+double orc_divb_linf(const OrcGrid* g, double* const* u8) { return divb_linf(to_grid(g), to_field(u8)); }
+double orc_divb_relative_linf(const OrcGrid* g, double* const* u8, int owned) {
+  return divb_relative_linf(to_grid(g), to_field(u8), owned != 0);
+}
+void orc_conserved_totals(const OrcGrid* g, double* const* u8, double* mass, double* energy) {
+  conserved_totals(to_grid(g), to_field(u8), mass, energy);
+}
+int orc_seed_initial_state(const OrcGrid* g, const OrcIcSpec* p, double* const* out8) {
+  IcSpec s;
+  s.kind = p->kind;
+  s.gamma = p->gamma;
+  s.magnetic_scale = p->magnetic_scale;
+  s.interface_x = p->interface_x;
+  for (int k = 0; k < 8; ++k) {
+    s.left[k] = p->left[k];
+    s.right[k] = p->right[k];
+  }
+  s.rho = p->rho;
+  s.pressure = p->pressure;
+  s.b0 = p->b0;
+  s.total_b0 = p->total_b0;
+  s.amplitude = p->amplitude;
+  s.wavenumber = p->wavenumber;
+  s.magnetic_velocity_scale = p->magnetic_velocity_scale;
+  for (int k = 0; k < 3; ++k) s.b_uniform[k] = p->b_uniform[k];
+  s.center[0] = p->center[0];
+  s.center[1] = p->center[1];
+  s.r_in = p->r_in;
+  s.r0 = p->r0;
+  s.r1 = p->r1;
+  s.rho_in = p->rho_in;
+  s.rho_out = p->rho_out;
+  s.p_in = p->p_in;
+  s.p_out = p->p_out;
+  s.p_core = p->p_core;
+  s.p_ambient = p->p_ambient;
+  s.rho_ambient = p->rho_ambient;
+  s.u0 = p->u0;
+  s.ot_rho = p->ot_rho;
+  s.ot_p = p->ot_p;
+  s.noise_amplitude = p->noise_amplitude;
+  s.noise_seed = p->noise_seed;
+  Field out = to_field(out8);
+  return seed_initial_state(to_grid(g), s, out);
+}
+
+void* orc_solver_create(const OrcGrid* g, double gamma, double cfl, int order,
+                        const int* fluid_flags, const int* field_flags) {
+  try {
+    return new Solver(to_grid(g), gamma, cfl, order, to_flags(fluid_flags), to_flags(field_flags));
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return nullptr;
+  }
+}
+void orc_solver_destroy(void* h) { delete static_cast<Solver*>(h); }
+int orc_solver_seed(void* h, int comp, const double* data) {
+  return guarded([&] { static_cast<Solver*>(h)->seed(comp, data); });
+}
+int orc_solver_get(void* h, int comp, double* out) {
+  return guarded([&] {
+    Solver* s = static_cast<Solver*>(h);
+    const auto& d = s->live().data[comp];
+    std::memcpy(out, d.data(), d.size() * sizeof(double));
+  });
+}
+int orc_solver_get_collocated(void* h, int axis, double* out) {
+  return guarded([&] { static_cast<Solver*>(h)->collocated(axis, out); });
+}
+int orc_solver_cfl_limit(void* h, double* dt) {
+  return guarded([&] { *dt = static_cast<Solver*>(h)->cfl_limit(0); });
+}
+int orc_solver_step_unchecked(void* h, double dt) {
+  return guarded([&] { static_cast<Solver*>(h)->step_unchecked(dt); });
+}
+int orc_solver_step(void* h, double dt) {
+  return guarded([&] { static_cast<Solver*>(h)->step(dt); });
+}
+int orc_solver_divergence_b_max(void* h, double* out) {
+  return guarded([&] { *out = static_cast<Solver*>(h)->divergence_b_max(); });
+}
+int orc_solver_conserved_totals(void* h, double* mass, double* energy) {
+  return guarded([&] { static_cast<Solver*>(h)->totals(mass, energy); });
+}
+int orc_solver_compute_residual(void* h, double* const* dudt8) {
+  return guarded([&] {
+    Field d = to_field(dudt8);
+    static_cast<Solver*>(h)->compute_residual_public(d);
+  });
+}
+int orc_solver_last_substeps(void* h) { return static_cast<Solver*>(h)->last_substeps; }
+void orc_solver_counters(void* h, OrcCounters* out) {
+  std::memset(out, 0, sizeof(*out));
+  copy_counters(static_cast<Solver*>(h)->counters, out);
+}
+const char* orc_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

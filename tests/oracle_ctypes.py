@@ -1,0 +1,433 @@
+"""ctypes bindings to the oracle (oracle/_build/liboracle.so) and to the
+reference's header-only numerics built in place (oracle/_ref/libquasar_ref.so).
+
+Test infrastructure only: the oracle is the checker, never the thing measured.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "_build" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libquasar_ref.so"
+
+D = C.c_double
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("ng", C.c_int32), ("pad_", C.c_int32),
+                ("lx", D), ("ly", D), ("ox", D), ("oy", D), ("cell_dx", D), ("cell_dy", D)]
+
+
+class OrcIcSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32),
+                ("gamma", D), ("magnetic_scale", D), ("interface_x", D),
+                ("left", D * 8), ("right", D * 8),
+                ("rho", D), ("pressure", D), ("b0", D), ("total_b0", D), ("amplitude", D),
+                ("wavenumber", C.c_int64), ("magnetic_velocity_scale", D),
+                ("b_uniform", D * 3), ("center", D * 2),
+                ("r_in", D), ("r0", D), ("r1", D), ("rho_in", D), ("rho_out", D),
+                ("p_in", D), ("p_out", D), ("p_core", D), ("p_ambient", D),
+                ("rho_ambient", D), ("u0", D),
+                ("ot_rho", D), ("ot_p", D), ("noise_amplitude", D), ("noise_seed", C.c_uint64)]
+
+
+class OrcCounters(C.Structure):
+    _fields_ = [("mp_clips", C.c_longlong), ("muscl_fallbacks", C.c_longlong),
+                ("pc_fallbacks", C.c_longlong), ("sentinels", C.c_longlong),
+                ("point_fallbacks", C.c_longlong), ("hll_fallbacks", C.c_longlong)]
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(DP)
+
+
+def _planes(arrs):
+    arr = (C.c_void_p * len(arrs))()
+    for k, a in enumerate(arrs):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        arr[k] = a.ctypes.data
+    return arr
+
+
+def _flags(f):
+    return (C.c_int * 4)(*f)
+
+
+class Oracle:
+    def __init__(self, path: os.PathLike = ORACLE_SO):
+        self.lib = C.CDLL(str(path))
+        L = self.lib
+        L.orc_cr_sum.restype = D
+        L.orc_product_sum.restype = D
+        L.orc_pressure.restype = D
+        L.orc_fast_speed.restype = D
+        for n in ("orc_mp_limit", "orc_mp5_interp", "orc_mp7_interp", "orc_mp5_reconstruct",
+                  "orc_mp7_reconstruct", "orc_philox_uniform_pm1", "orc_cell_bx", "orc_cell_by",
+                  "orc_cfl_max_rate", "orc_admissible_fraction", "orc_divb_linf",
+                  "orc_divb_relative_linf", "orc_point_value", "orc_gauss_average"):
+            getattr(L, n).restype = D
+        L.orc_mp_limit.argtypes = [D, DP]
+        L.orc_philox_uniform_pm1.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
+        L.orc_hll_flux_x.argtypes = [DP, DP, D, D, D, DP]
+        L.orc_lf_flux_x.argtypes = [DP, DP, D, DP]
+        L.orc_hlld_flux_x.argtypes = [DP, DP, D, DP]
+        L.orc_physical_flux_x.argtypes = [DP, D, DP]
+        L.orc_pressure.argtypes = [DP, D]
+        L.orc_fast_speed.argtypes = [DP, C.c_int, D]
+        L.orc_cell_bx.argtypes = [C.POINTER(OrcGrid), DP, C.c_int, C.c_int, C.c_int]
+        L.orc_cell_by.argtypes = [C.POINTER(OrcGrid), DP, C.c_int, C.c_int, C.c_int]
+        L.orc_rk_stage.argtypes = [C.POINTER(OrcGrid), D, D, D, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]
+        L.orc_admissible_fraction.argtypes = [C.POINTER(OrcGrid), C.c_void_p, C.c_void_p, D, D,
+                                              C.c_int]
+        L.orc_cfl_max_rate.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_void_p, D, C.c_void_p,
+                                       C.c_void_p]
+        L.orc_solver_create.restype = C.c_void_p
+        L.orc_solver_create.argtypes = [C.POINTER(OrcGrid), D, D, C.c_int, C.c_void_p, C.c_void_p]
+        for n in ("orc_solver_destroy", "orc_solver_seed", "orc_solver_get",
+                  "orc_solver_get_collocated", "orc_solver_cfl_limit",
+                  "orc_solver_step_unchecked", "orc_solver_step", "orc_solver_divergence_b_max",
+                  "orc_solver_conserved_totals", "orc_solver_compute_residual",
+                  "orc_solver_last_substeps", "orc_solver_counters"):
+            getattr(L, n).argtypes = None
+        L.orc_solver_destroy.argtypes = [C.c_void_p]
+        L.orc_solver_seed.argtypes = [C.c_void_p, C.c_int, DP]
+        L.orc_solver_get.argtypes = [C.c_void_p, C.c_int, DP]
+        L.orc_solver_get_collocated.argtypes = [C.c_void_p, C.c_int, DP]
+        L.orc_solver_cfl_limit.argtypes = [C.c_void_p, DP]
+        L.orc_solver_step_unchecked.argtypes = [C.c_void_p, D]
+        L.orc_solver_step.argtypes = [C.c_void_p, D]
+        L.orc_solver_divergence_b_max.argtypes = [C.c_void_p, DP]
+        L.orc_solver_conserved_totals.argtypes = [C.c_void_p, DP, DP]
+        L.orc_solver_compute_residual.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_solver_last_substeps.argtypes = [C.c_void_p]
+        L.orc_solver_counters.argtypes = [C.c_void_p, C.POINTER(OrcCounters)]
+        L.orc_last_error.restype = C.c_char_p
+
+    # -- scalar numerics --------------------------------------------------
+    def cr_sum(self, terms):
+        t = np.ascontiguousarray(terms, dtype=np.float64)
+        return self.lib.orc_cr_sum(_ptr(t), len(t))
+
+    def product_sum(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        return self.lib.orc_product_sum(_ptr(a), _ptr(b), len(a))
+
+    def pressure(self, u, gamma):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return self.lib.orc_pressure(_ptr(u), gamma)
+
+    def fast_speed(self, u, d, gamma):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return self.lib.orc_fast_speed(_ptr(u), d, gamma)
+
+    def physical_flux_x(self, u, gamma):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        f = np.zeros(8)
+        self.lib.orc_physical_flux_x(_ptr(u), gamma, _ptr(f))
+        return f
+
+    def hll_flux_x(self, l, r, sl, sr, gamma):
+        l = np.ascontiguousarray(l, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        f = np.zeros(8)
+        self.lib.orc_hll_flux_x(_ptr(l), _ptr(r), sl, sr, gamma, _ptr(f))
+        return f
+
+    def lf_flux_x(self, l, r, gamma):
+        l = np.ascontiguousarray(l, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        f = np.zeros(8)
+        self.lib.orc_lf_flux_x(_ptr(l), _ptr(r), gamma, _ptr(f))
+        return f
+
+    def hlld_flux_x(self, l, r, gamma, return_path=False):
+        l = np.ascontiguousarray(l, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        f = np.zeros(8)
+        path = self.lib.orc_hlld_flux_x(_ptr(l), _ptr(r), gamma, _ptr(f))
+        return (f, path) if return_path else f
+
+    def mp_limit(self, vl, v5):
+        v = np.ascontiguousarray(v5, dtype=np.float64)
+        return self.lib.orc_mp_limit(vl, _ptr(v))
+
+    def mp5_interp(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        return self.lib.orc_mp5_interp(_ptr(v))
+
+    def mp7_interp(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        return self.lib.orc_mp7_interp(_ptr(v))
+
+    def mp5_reconstruct(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        return self.lib.orc_mp5_reconstruct(_ptr(v))
+
+    def mp7_reconstruct(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        return self.lib.orc_mp7_reconstruct(_ptr(v))
+
+    def eigen_build(self, u, gamma):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        L = np.zeros(49)
+        R = np.zeros(49)
+        s = np.zeros(7)
+        fb = self.lib.orc_eigen_build(_ptr(u), D(gamma), _ptr(L), _ptr(R), _ptr(s))
+        return L.reshape(7, 7), R.reshape(7, 7), s, fb
+
+    def point_value(self, order, node, samples):
+        s = np.ascontiguousarray(samples, dtype=np.float64)
+        return self.lib.orc_point_value(order, node, _ptr(s))
+
+    def gauss_average(self, order, samples):
+        s = np.ascontiguousarray(samples, dtype=np.float64)
+        return self.lib.orc_gauss_average(order, _ptr(s))
+
+    def philox(self, seed, comp, cell):
+        return self.lib.orc_philox_uniform_pm1(seed, comp, cell)
+
+    # -- grid kernels -----------------------------------------------------
+    @staticmethod
+    def grid(nx, ny, ng, lx=1.0, ly=1.0, ox=0.0, oy=0.0):
+        return OrcGrid(nx, ny, ng, 0, lx, ly, ox, oy, 0.0, 0.0)
+
+    def cell_bx(self, g, face, i, j, co=0):
+        return self.lib.orc_cell_bx(C.byref(g), _ptr(face), i, j, co)
+
+    def cell_by(self, g, face, i, j, co=0):
+        return self.lib.orc_cell_by(C.byref(g), _ptr(face), i, j, co)
+
+    def reconstruct(self, g, d, order, flags, gamma, u8, out16):
+        c = OrcCounters()
+        self.lib.orc_reconstruct(C.byref(g), d, order, _flags(flags), D(gamma), _planes(u8),
+                                 _planes(out16), C.byref(c))
+        return c
+
+    def face_flux(self, g, d, order, hll_only, flags, gamma, in16, flux8):
+        c = OrcCounters()
+        self.lib.orc_face_flux(C.byref(g), d, order, hll_only, _flags(flags), D(gamma),
+                               _planes(in16), _planes(flux8), C.byref(c))
+        return c
+
+    def flux_difference(self, g, d, flux8, dudt8):
+        self.lib.orc_flux_difference(C.byref(g), d, _planes(flux8), _planes(dudt8))
+
+    def ct_emf(self, g, order, hll_only, flags, gamma, u8, ifx16, ify16):
+        n = len(u8[0])
+        cell = np.zeros(n)
+        xf = np.zeros(n)
+        yf = np.zeros(n)
+        ez = np.zeros(n)
+        xnj = np.zeros(n, dtype=np.int32)
+        ynj = np.zeros(n, dtype=np.int32)
+        self.lib.orc_ct_emf(C.byref(g), order, hll_only, _flags(flags), D(gamma), _planes(u8),
+                            _planes(ifx16), _planes(ify16), _ptr(cell), _ptr(xf), _ptr(yf),
+                            xnj.ctypes.data_as(IP), ynj.ctypes.data_as(IP), _ptr(ez))
+        return dict(cell_ez=cell, xface_ez=xf, yface_ez=yf, xnj=xnj, ynj=ynj, ez=ez)
+
+    def emf_curl_rate(self, g, ez, dudt8):
+        self.lib.orc_emf_curl_rate(C.byref(g), _ptr(ez), _planes(dudt8))
+
+    def rk_stage(self, g, a, b, c, un8, us8, dudt8, out8):
+        self.lib.orc_rk_stage(C.byref(g), a, b, c, _planes(un8), _planes(us8), _planes(dudt8),
+                              _planes(out8))
+
+    def fill_ghosts(self, g, fluid, field, u8):
+        self.lib.orc_fill_ghosts(C.byref(g), _flags(fluid), _flags(field), _planes(u8))
+
+    def cfl_max_rate(self, g, order, flags, gamma, ifx16, ify16):
+        return self.lib.orc_cfl_max_rate(C.byref(g), order, _flags(flags), gamma, _planes(ifx16),
+                                         _planes(ify16))
+
+    def admissible_fraction(self, g, base8, cand8, co=0):
+        return self.lib.orc_admissible_fraction(C.byref(g), _planes(base8), _planes(cand8), 0.0,
+                                                0.0, co)
+
+    def divb_linf(self, g, u8):
+        return self.lib.orc_divb_linf(C.byref(g), _planes(u8))
+
+    def divb_relative_linf(self, g, u8, owned):
+        return self.lib.orc_divb_relative_linf(C.byref(g), _planes(u8), int(owned))
+
+    def conserved_totals(self, g, u8):
+        m = D()
+        e = D()
+        self.lib.orc_conserved_totals(C.byref(g), _planes(u8), C.byref(m), C.byref(e))
+        return m.value, e.value
+
+    def seed(self, g, spec: OrcIcSpec):
+        n = (g.nx + 2 * g.ng) * (g.ny + 2 * g.ng)
+        out = [np.zeros(n) for _ in range(8)]
+        status = self.lib.orc_seed_initial_state(C.byref(g), C.byref(spec), _planes(out))
+        return status, out
+
+    def last_error(self):
+        return self.lib.orc_last_error().decode()
+
+
+def ic_spec(kind: str, gamma: float, **kw) -> OrcIcSpec:
+    kinds = {"brio_wu": 0, "alfven_wave": 1, "orszag_tang": 2, "blast": 3, "rotor": 4}
+    s = OrcIcSpec()
+    s.kind = kinds[kind]
+    s.gamma = gamma
+    s.magnetic_scale = 1.0
+    s.interface_x = 0.5
+    s.rho, s.pressure, s.b0, s.total_b0, s.amplitude = 1.0, 0.1, 1.0, 1.0, 1e-6
+    s.wavenumber = 1
+    s.magnetic_velocity_scale = 1.0
+    s.r_in, s.r0, s.r1, s.rho_in, s.rho_out = 0.1, 0.1, 0.115, 10.0, 1.0
+    s.p_in, s.p_out, s.p_core, s.p_ambient, s.rho_ambient, s.u0 = 1.0, 0.1, 10.0, 0.1, 1.0, 2.0
+    for k, v in kw.items():
+        if k in ("left", "right"):
+            getattr(s, k)[:] = v
+        elif k in ("b_uniform", "center"):
+            getattr(s, k)[:] = v
+        else:
+            setattr(s, k, v)
+    return s
+
+
+class OracleSolver:
+    """Restatement of MhdSolver2D driven through the oracle's C ABI."""
+
+    def __init__(self, oracle: Oracle, g: OrcGrid, gamma: float, cfl: float, order: int,
+                 fluid=(0, 0, 0, 0), field=None):
+        self.o = oracle
+        self.g = g
+        field = fluid if field is None else field
+        self._fluid = _flags(fluid)
+        self._field = _flags(field)
+        self.h = oracle.lib.orc_solver_create(C.byref(g), gamma, cfl, order, self._fluid,
+                                              self._field)
+        if not self.h:
+            raise RuntimeError(oracle.last_error())
+        self.n = (g.nx + 2 * g.ng) * (g.ny + 2 * g.ng)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.o.lib.orc_solver_destroy(self.h)
+            self.h = None
+
+    def _check(self, status):
+        if status == 1:
+            raise ValueError(self.o.last_error())
+        if status != 0:
+            raise RuntimeError(self.o.last_error())
+
+    def seed(self, planes):
+        for c, p in enumerate(planes):
+            p = np.ascontiguousarray(p, dtype=np.float64)
+            self._check(self.o.lib.orc_solver_seed(self.h, c, _ptr(p)))
+
+    def state(self):
+        out = []
+        for c in range(8):
+            a = np.zeros(self.n)
+            self._check(self.o.lib.orc_solver_get(self.h, c, _ptr(a)))
+            out.append(a)
+        return out
+
+    def cfl_limit(self):
+        dt = D()
+        self._check(self.o.lib.orc_solver_cfl_limit(self.h, C.byref(dt)))
+        return dt.value
+
+    def step_unchecked(self, dt):
+        self._check(self.o.lib.orc_solver_step_unchecked(self.h, D(dt)))
+
+    def step(self, dt):
+        self._check(self.o.lib.orc_solver_step(self.h, D(dt)))
+
+    def divergence_b_max(self):
+        v = D()
+        self._check(self.o.lib.orc_solver_divergence_b_max(self.h, C.byref(v)))
+        return v.value
+
+    def conserved_totals(self):
+        m, e = D(), D()
+        self._check(self.o.lib.orc_solver_conserved_totals(self.h, C.byref(m), C.byref(e)))
+        return m.value, e.value
+
+    def compute_residual(self):
+        out = [np.zeros(self.n) for _ in range(8)]
+        self._check(self.o.lib.orc_solver_compute_residual(self.h, _planes(out)))
+        return out
+
+    def counters(self):
+        c = OrcCounters()
+        self.o.lib.orc_solver_counters(self.h, C.byref(c))
+        return c
+
+    def last_substeps(self):
+        return self.o.lib.orc_solver_last_substeps(self.h)
+
+
+class Reference:
+    """The reference's header-only numerics, compiled in place (oracle/_ref)."""
+
+    def __init__(self, path: os.PathLike = REF_SO):
+        self.lib = C.CDLL(str(path))
+        L = self.lib
+        for n in ("ref_product_sum", "ref_pressure", "ref_fast_speed", "ref_mp_limit",
+                  "ref_mp5_interp", "ref_mp7_interp", "ref_mp5_reconstruct",
+                  "ref_mp7_reconstruct", "ref_cell_bx", "ref_cell_by", "ref_point_value7",
+                  "ref_point_value5", "ref_gauss_average", "ref_x_at_cell_center"):
+            getattr(L, n).restype = D
+        L.ref_mp_limit.argtypes = [D, DP]
+        L.ref_hll_flux_x.argtypes = [DP, DP, D, D, D, DP]
+        L.ref_lf_flux_x.argtypes = [DP, DP, D, DP]
+        L.ref_hlld_flux_x.argtypes = [DP, DP, D, DP]
+        L.ref_physical_flux_x.argtypes = [DP, D, DP]
+        L.ref_pressure.argtypes = [DP, D]
+        L.ref_fast_speed.argtypes = [DP, C.c_int, D]
+        L.ref_eigen_build.argtypes = [DP, D, DP, DP, DP]
+        L.ref_to_char.argtypes = [DP, DP, D, DP]
+        L.ref_from_char.argtypes = [DP, DP, D, DP]
+        L.ref_cell_bx.argtypes = [C.c_int, C.c_int, C.c_int, D, D, D, D, DP, C.c_int, C.c_int,
+                                  C.c_int]
+        L.ref_cell_by.argtypes = L.ref_cell_bx.argtypes
+        L.ref_point_value7.argtypes = [C.c_int, DP]
+        L.ref_point_value5.argtypes = [C.c_int, DP]
+        L.ref_gauss_average.argtypes = [C.c_int, DP]
+        L.ref_x_at_cell_center.argtypes = [C.c_int, D, D, C.c_int]
+
+    def call8(self, name, *args):
+        f = np.zeros(8)
+        conv = [(_ptr(np.ascontiguousarray(a, dtype=np.float64)) if isinstance(a, np.ndarray)
+                 else a) for a in args]
+        getattr(self.lib, name)(*conv, _ptr(f))
+        return f
+
+    def scalar(self, name, *args):
+        conv = [(_ptr(np.ascontiguousarray(a, dtype=np.float64)) if isinstance(a, np.ndarray)
+                 else a) for a in args]
+        return getattr(self.lib, name)(*conv)
+
+    def eigen_build(self, u, gamma):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        L = np.zeros(49)
+        R = np.zeros(49)
+        s = np.zeros(7)
+        self.lib.ref_eigen_build(_ptr(u), gamma, _ptr(L), _ptr(R), _ptr(s))
+        return L.reshape(7, 7), R.reshape(7, 7), s
+
+
+def reference_available() -> bool:
+    return REF_SO.exists()
+
+
+def conserved(rho, vx, vy, vz, p, bx, by, bz, gamma):
+    """Primitive -> conserved in the reference's test helper form."""
+    e_int = p / ((gamma - 1.0) * rho)
+    e = rho * e_int + 0.5 * rho * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz)
+    return np.array([rho, rho * vx, rho * vy, rho * vz, e, bx, by, bz])
