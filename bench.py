@@ -1,0 +1,256 @@
+#!/usr/bin/env python3
+"""Zone-update throughput of the FP64 constrained-transport MHD step on one MI355X.
+
+Workload: Orszag-Tang vortex, 2048^2 periodic cells on [0,1]^2, gamma 5/3,
+rho = 25/(36 pi), p = 5/(12 pi), B0 = 1/sqrt(4 pi); MP7 characteristic
+reconstruction + HLLD + upwind constrained transport, SSP-RK3, CFL 0.4 (0.2
+effective for order >= 5), timestep from the device CFL reduction each step.
+The state is seeded on the device from the closed-form initial condition and
+stays resident in HBM; setup is outside the timed region.
+
+One zone-update = one cell advanced by one full three-stage step, including
+the CFL reduction, the positivity/solenoidality checks and the ghost fills
+(python/quasar/mhd/cli.py:386-491 is the loop being timed).
+
+Prints one JSON line.  `roofline` describes the kernel with the largest share
+of the timed region: its average launch duration comes from HIP events that
+the solver records around each launch on its own stream during the timed
+steps.  `cpu_baseline` times the CPU restatement of the same step (OpenMP
+C++) on a bounded sample of the same problem.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+
+# Algorithmic cost per unit of each launch kind, FP64.  A unit is one face of
+# the launch's direction for reconstruct/flux (the launch covers (n+1) faces
+# per row over the interior rows), one interior cell for the others.
+#   bytes: every plane the kernel must read once and write once (8 B words),
+#   flops: the arithmetic of the algorithm for one unit (add/mul/div/sqrt
+#          count 1, fma 2), from the per-face operation counts of the
+#          characteristic MP7 reconstruction, 4-point transverse HLLD, the
+#          4x4 tensor-Gauss cell EMF and the 8-tap corner interpolation.
+UNIT_COST = {
+    # 8 state planes + 2 collocated field planes in, 16 interface planes + 1 flag out
+    "reconstruct": {"bytes": 8 * (8 + 2 + 16) + 4, "flops": 6440},
+    # 16 interface planes in, 8 flux planes + 1 rate plane out
+    "flux": {"bytes": 8 * (16 + 8 + 1), "flops": 3520},
+    # 8 state planes + 2 collocated field planes in, 1 EMF plane out
+    "cell_emf": {"bytes": 8 * (8 + 2 + 1), "flops": 4610},
+    # fx.by, fy.bx, cell EMF, 2 flag planes in, corner EMF out
+    "corner": {"bytes": 8 * (3 + 1) + 2 * 4, "flops": 180},
+    # 8 base + 8 stage + 8+8 flux + 1 EMF planes in, 8 planes out
+    "update": {"bytes": 8 * (8 + 8 + 16 + 1 + 8), "flops": 90},
+    # 8 candidate + 8 base planes in, 2 collocated field planes out
+    "prep": {"bytes": 8 * (8 + 8 + 2), "flops": 260},
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=2048, help="cells per side")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="bound on the CPU baseline's timed work")
+    ap.add_argument("--cpu-n", type=int, default=256, help="cells per side of the CPU sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the timed region, copy a buffer of this size once (HBM traffic calibration "
+                         "for counter runs)")
+    return ap.parse_args()
+
+
+def orszag_tang(n: int):
+    from quasar_amd import mhd
+    spec = mhd.MhdInitialConditionSpec()
+    spec.kind = "orszag_tang"
+    spec.grid = mhd.Grid2D(n, n, 1.0, 1.0, 0.0, 0.0, 4)
+    spec.gamma = 5.0 / 3.0
+    spec.b_uniform = [1.0 / math.sqrt(4.0 * math.pi), 0.0, 0.0]
+    spec.ot_rho = 25.0 / (36.0 * math.pi)
+    spec.ot_p = 5.0 / (12.0 * math.pi)
+    return spec
+
+
+def make_solver(n: int):
+    from quasar_amd import mhd
+    spec = orszag_tang(n)
+    cfg = mhd.MhdConfig()
+    cfg.grid = spec.grid
+    cfg.gamma = spec.gamma
+    cfg.reconstruction = "mp7"
+    cfg.cfl = 0.4
+    cfg.boundary.set_fluid_all("periodic")
+    cfg.boundary.set_field_all("periodic")
+    solver = mhd.MhdSolver2D(cfg)
+    state = mhd.build_initial_state(spec)
+    for name, plane in state.items():
+        solver.seed_state(name, plane)
+    return solver
+
+
+def units(kind: str, n: int) -> int:
+    if kind in ("reconstruct", "flux"):
+        return (n + 1) * n
+    return n * n
+
+
+def roofline(solver, n: int):
+    times = solver.kernel_times()
+    kind = max((k for k in UNIT_COST), key=lambda k: times[k][0])
+    total_ms, launches = times[kind]
+    avg_s = total_ms / launches / 1e3
+    cost = UNIT_COST[kind]
+    u = units(kind, n)
+    gbs = cost["bytes"] * u / avg_s / 1e9
+    tfs = cost["flops"] * u / avg_s / 1e12
+    intensity = cost["flops"] / cost["bytes"]
+    balance = PEAK_FP64_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9)
+    traffic = measured_traffic(kind, n)
+    if intensity > balance:
+        r = {"bound": "mfma", "achieved": round(tfs, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+             "frac": round(tfs / PEAK_FP64_TFLOPS, 4)}
+    else:
+        r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+             "frac": round(gbs / PEAK_HBM_GBS, 4)}
+    r["traffic"] = traffic
+    r["kernel"] = kind
+    r["avg_launch_ms"] = round(total_ms / launches, 4)
+    r["launches"] = launches
+    r["algorithmic_bytes_per_launch"] = cost["bytes"] * u
+    r["algorithmic_flops_per_launch"] = cost["flops"] * u
+    r["fp64_peak_note"] = "MI355X FP64 vector and matrix peak are both 78.6 TFLOP/s; kernels use FP64 VALU"
+    shares = {k: round(v[0], 3) for k, v in times.items()}
+    return r, shares
+
+
+def measured_traffic(kind: str, n: int):
+    """HBM bytes per launch from the committed counter summary, when one exists
+    for this kernel and size."""
+    p = ROOT / "profiles" / f"traffic_ot{n}_mp7.json"
+    if not p.exists():
+        return None
+    try:
+        data = json.loads(p.read_text())
+        return data.get(kind)
+    except (ValueError, OSError):
+        return None
+
+
+def cpu_baseline(n: int, budget_s: float):
+    """The CPU restatement (OpenMP C++) on Orszag-Tang n^2 MP7 FP64: steps are
+    taken until `budget_s` seconds of CPU work have been timed."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_ctypes import Oracle, OracleSolver, ic_spec
+    oracle = Oracle()
+    g = oracle.grid(n, n, 4)
+    spec = ic_spec("orszag_tang", 5.0 / 3.0, b_uniform=[1 / math.sqrt(4 * math.pi), 0, 0],
+                   ot_rho=25 / (36 * math.pi), ot_p=5 / (12 * math.pi))
+    status, planes = oracle.seed(g, spec)
+    if status != 0:
+        raise RuntimeError("oracle seed failed")
+    ref = OracleSolver(oracle, g, 5.0 / 3.0, 0.4, 7)
+    ref.seed(planes)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        ref.step(ref.cfl_limit())
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 1000:
+            break
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return {"value": round(n * n * steps / el, 1), "unit": "zone-updates/s", "cores": cores, "kind": "port",
+            "sample": f"Orszag-Tang {n}^2 MP7+HLLD+CT FP64, {steps} steps ({el:.1f} s), C++ restatement "
+                      f"with OpenMP on {cores} host threads"}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the step is not sharded across GPUs", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    from quasar_amd import _lib
+    _lib.load()  # raises when the gfx950 library is missing: there is no CPU path
+
+    n = args.n
+    print(f"bench: seeding Orszag-Tang {n}^2 MP7 FP64", file=sys.stderr, flush=True)
+    solver = make_solver(n)
+    if args.copy_probe_mib > 0:
+        a = torch.ones(args.copy_probe_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    if args.warmup > 0:
+        solver.run(args.warmup)
+    print(f"bench: warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    solver.set_profiling(True)
+    solver.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done, sim_t = solver.run(args.steps)
+    solver.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if done != args.steps:
+        raise RuntimeError(f"run stopped after {done} of {args.steps} steps")
+    roof, shares = roofline(solver, n)
+    solver.set_profiling(False)
+    print(f"bench: {done} steps, wall {wall:.3f} s, kernels {sum(shares.values()) / 1e3:.3f} s", file=sys.stderr,
+          flush=True)
+
+    zone_updates = n * n * done
+    value = zone_updates / wall
+    line = {
+        "metric": "zone-updates/s, 1 MI355X: FP64 plain vs FP32+learned at fixed L1 err; %HBM",
+        "value": round(value, 1),
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / done * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: Orszag-Tang initial condition from its closed form",
+        "config": {"workload": f"orszag_tang_{n}x{n}_mp7_hlld_ct_ssprk3_fp64", "form": "plain FP64",
+                   "cfl": 0.4, "boundaries": "periodic", "sim_time": sim_t},
+        "hbm_frac_step": round(value * 512 / (PEAK_HBM_GBS * 1e9), 5),
+        "kernel_ms": shares,
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,214 @@
+/* quasar_mhd.h — C ABI of the MI355X (gfx950) implementation of quasar's 2D
+ * constrained-transport ideal-MHD step.
+ *
+ * Two layers, both plain C: pointers, sizes and status codes, no exceptions
+ * and no framework types cross it.
+ *
+ *  1. Launch level: one entry point per operation of the reference launch ABI
+ *     (include/quasar/physics/mhd/kernels.hpp, the launch_mhd_* family).
+ *     Buffers are caller-owned device arrays of `qmhd_grid_storage(g)` doubles
+ *     in the reference's padded Grid2D layout, index (i+ng) + (nx+2ng)*(j+ng);
+ *     a state is 8 planes in the order rho, mx, my, mz, energy, bx_face,
+ *     by_face, bz_cell; interface states are 16 planes (L then R).  Work
+ *     memory is caller-owned too: `scratch` must hold
+ *     qmhd_scratch_doubles(g) doubles.  `stream` is a hipStream_t (NULL = the
+ *     default stream); calls are stream ordered, and those that return a host
+ *     scalar synchronize that stream.
+ *
+ *  2. Solver level: an opaque handle with the seed / cfl_limit / step /
+ *     readback surface of quasar::mhd::MhdSolver2D
+ *     (include/quasar/physics/mhd/mhd_solver.hpp:85-173) that the pybind
+ *     module _core.mhd exposes (bindings/python/bind_mhd.cpp:571-646).
+ *
+ * Status codes: QMHD_OK, QMHD_EINVAL (the reference throws
+ * std::invalid_argument), QMHD_ERUNTIME (std::runtime_error), QMHD_EHIP (a HIP
+ * runtime error; the reference throws backend::DeviceError).
+ * qmhd_last_error() returns the message of the calling thread's last failure.
+ */
+#ifndef QUASAR_MHD_H
+#define QUASAR_MHD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { QMHD_OK = 0, QMHD_EINVAL = 1, QMHD_ERUNTIME = 2, QMHD_EHIP = 3 };
+
+/* Side boundary modes (kernels.hpp:43-51): 0 periodic, 1 outflow, 2 wall,
+ * 3 axis, 4 internal.  Side order x_lo, x_hi, y_lo, y_hi. */
+
+/* Mirrors quasar::Grid2D (include/quasar/core/grid.hpp:84-200). */
+typedef struct {
+  int32_t nx, ny, ng, reserved;
+  double lx, ly, origin_x, origin_y, cell_dx, cell_dy;
+} qmhd_grid;
+
+/* Mirrors quasar::mhd::MhdInitialConditionSpec
+ * (include/quasar/physics/mhd/initial_conditions.hpp:62-124), kind 0 brio_wu,
+ * 1 alfven_wave, 2 orszag_tang, 3 blast, 4 rotor.  Extensions: ot_rho/ot_p
+ * (Orszag-Tang background density/pressure; 0 keeps gamma^2/gamma) and a
+ * seeded Philox-4x32-10 white-noise perturbation of amplitude
+ * noise_amplitude on rho (relative), vx and vy. */
+typedef struct {
+  int32_t kind, reserved;
+  double gamma, magnetic_scale, interface_x;
+  double left[8], right[8];
+  double rho, pressure, b0, total_b0, amplitude;
+  int64_t wavenumber;
+  double magnetic_velocity_scale;
+  double b_uniform[3];
+  double center[2];
+  double r_in, r0, r1, rho_in, rho_out, p_in, p_out, p_core, p_ambient, rho_ambient, u0;
+  double ot_rho, ot_p;
+  double noise_amplitude;
+  uint64_t noise_seed;
+} qmhd_ic_spec;
+
+const char* qmhd_last_error(void);
+const char* qmhd_version(void);
+size_t qmhd_grid_storage(const qmhd_grid* g);
+size_t qmhd_scratch_doubles(const qmhd_grid* g);
+
+/* ---- launch level ---------------------------------------------------- */
+
+/* launch_mhd_collocate_face_to_cell (kernels.hpp:735-738): face -> cell
+ * average of bx_face (axis 0) or by_face (axis 1); collocation 0 = the
+ * high-order rule of the grid's halo, 1 = the two-face average. */
+int qmhd_collocate_face_to_cell(const qmhd_grid* g, int axis, int collocation, const double* face,
+                                double* cell, void* stream);
+
+/* launch_mhd_reconstruct (kernels.hpp:162-166): L/R interface states normal to
+ * `dir` for order 1 (piecewise constant), 2 (MUSCL), 5 (MP5), 7 (MP7).
+ * Writes out16 on dir 0 faces i in [0,nx], j in [-ng, ny+ng) (dir 1
+ * transposed). */
+int qmhd_reconstruct(const qmhd_grid* g, int dir, int order, const int32_t* flags4, double gamma,
+                     const double* const* u8, double* const* out16, double* scratch, void* stream);
+
+/* launch_mhd_hlld_flux (kernels.hpp:186-194): transverse-Gauss HLLD face flux
+ * (Lax-Friedrichs when hll_only) on faces [0,nx] x [0,ny]. */
+int qmhd_hlld_flux(const qmhd_grid* g, int dir, int order, int hll_only, const int32_t* flags4, double gamma,
+                   const double* const* in16, double* const* flux8, double* scratch, void* stream);
+
+/* launch_mhd_flux_difference (kernels.hpp:203-205): dudt += -(F(i+1)-F(i))/d. */
+int qmhd_flux_difference(const qmhd_grid* g, int dir, const double* const* flux8, double* const* dudt8,
+                         void* stream);
+
+/* launch_mhd_ct_emf (kernels.hpp:274-291): corner E_z from the cell-average
+ * E_z of u and the directional face EMFs of the two flux sets. */
+int qmhd_ct_emf(const qmhd_grid* g, int order, const int32_t* flags4, double gamma, const double* const* u8,
+                const double* const* ifx16, const double* const* ify16, const double* const* flux_x8,
+                const double* const* flux_y8, double* ez_edge, double* scratch, void* stream);
+
+/* launch_mhd_emf_curl_rate (kernels.hpp:307-309): face-B rates from E_z. */
+int qmhd_emf_curl_rate(const qmhd_grid* g, const double* ez_edge, double* dbx_face, double* dby_face,
+                       void* stream);
+
+/* launch_mhd_rk_stage (kernels.hpp:336-338): out = a un + b us + c dudt on
+ * cells and physical faces; (1,0,0) is an exact copy. */
+int qmhd_rk_stage(const qmhd_grid* g, double a, double b, double c, const double* const* un8,
+                  const double* const* us8, const double* const* dudt8, double* const* out8, void* stream);
+
+/* launch_mhd_fill_ghosts_fluid / _field (kernels.hpp:509-533). */
+int qmhd_fill_ghosts(const qmhd_grid* g, const int32_t* fluid_flags4, const int32_t* field_flags4,
+                     double* const* u8, void* stream);
+
+/* launch_mhd_cfl_max_rate (kernels.hpp:361-366): maximum additive face rate. */
+int qmhd_cfl_max_rate(const qmhd_grid* g, int order, const int32_t* flags4, double gamma,
+                      const double* const* ifx16, const double* const* ify16, double* host_rate,
+                      double* scratch, void* stream);
+
+/* launch_mhd_admissible_fraction (kernels.hpp:397-414). */
+int qmhd_admissible_fraction(const qmhd_grid* g, const double* const* base8, const double* const* cand8,
+                             int collocation, double* host_theta, double* scratch, void* stream);
+
+/* launch_mhd_ct_divb_linf / _relative_linf (kernels.hpp:483-495). */
+int qmhd_divb_linf(const qmhd_grid* g, const double* const* u8, double* host_linf, double* scratch,
+                   void* stream);
+int qmhd_divb_relative_linf(const qmhd_grid* g, const double* const* u8, int solver_owned, double* host_ratio,
+                            double* scratch, void* stream);
+
+/* launch_mhd_conserved_cell_sums (kernels.hpp:545-558): interior mass and
+ * energy totals (compensated device partials, exact host fold). */
+int qmhd_conserved_cell_sums(const qmhd_grid* g, const double* const* u8, double* mass, double* energy,
+                             double* scratch, void* stream);
+
+/* launch_mhd_seed_initial_state (kernels.hpp:656-660): analytic IC on every
+ * padded cell with the collocated magnetic energy. */
+int qmhd_seed_initial_state(const qmhd_grid* g, const qmhd_ic_spec* spec, double* const* out8, double* scratch,
+                            void* stream);
+
+/* ---- solver level (MhdSolver2D) ---------------------------------------- */
+
+typedef struct qmhd_solver qmhd_solver;
+
+/* MhdConfig (mhd_solver.hpp:50-83).  Scheme names are the reference's
+ * registry strings: reconstruction "muscl_minmod" | "mp5" | "mp7", riemann
+ * "hlld", integrator "ssprk3", ct "fd_ct_christlieb", positivity
+ * "troubled_cell", boundaries "periodic" | "outflow" | "wall".  grid.ng = 0
+ * lets the solver size the halo from the reconstruction. */
+typedef struct {
+  qmhd_grid grid;
+  double gamma, cfl, rho_floor, p_floor;
+  char reconstruction[32], riemann[32], integrator[32], ct[32], positivity[32], geometry[32];
+  char fluid[4][32];
+  char field[4][32];
+} qmhd_solver_config;
+
+int qmhd_solver_create(const qmhd_solver_config* cfg, qmhd_solver** out);
+void qmhd_solver_destroy(qmhd_solver* s);
+int qmhd_solver_grid(const qmhd_solver* s, qmhd_grid* out);
+/* seed_state / state_component_to_host: component "rho", "mx", "my", "mz",
+ * "energy", "bx"/"bx_face", "by"/"by_face", "bz"/"bz_cell"; readback of "bx"
+ * and "by" is face->cell collocated, "bx_face"/"by_face" raw. */
+int qmhd_solver_seed_state(qmhd_solver* s, const char* component, const double* host, size_t n);
+int qmhd_solver_state_to_host(qmhd_solver* s, const char* component, double* host, size_t n);
+int qmhd_solver_cfl_limit(qmhd_solver* s, double* dt);
+int qmhd_solver_step(qmhd_solver* s, double dt);
+int qmhd_solver_step_unchecked(qmhd_solver* s, double dt);
+int qmhd_solver_advance(qmhd_solver* s, double t_end, double dt);
+int qmhd_solver_divergence_b_max(qmhd_solver* s, double* out);
+int qmhd_solver_conserved_cell_totals(qmhd_solver* s, double* mass, double* energy);
+int qmhd_solver_last_positivity_substeps(const qmhd_solver* s);
+/* The auto-dt run loop (python/quasar/mhd/cli.py:386-491) with the timestep
+ * kept on the device: up to `steps` steps or until t_end (t_end <= 0: no end
+ * time), one host synchronization per step.  Returns steps taken and the
+ * simulated time. */
+int qmhd_solver_run(qmhd_solver* s, int steps, double t_end, int* steps_done, double* sim_time);
+/* Device pointer of a live-state plane (0..7), for zero-copy consumers. */
+int qmhd_solver_state_plane(qmhd_solver* s, int component, double** device_ptr);
+
+/* Launch timing.  No counterpart in the reference (which has no in-process
+ * kernel timing); for benchmark harnesses that need per-kernel durations on
+ * the solver's own stream.  While enabled, every launch of the step is
+ * bracketed by HIP events; qmhd_solver_kernel_time returns the summed
+ * milliseconds and launch count of one kind since enabling.  A ghost fill
+ * (its x and y launches) is one interval of kind GHOSTS; REDUCE covers the
+ * CFL reduction and the device dt kernel. */
+enum {
+  QMHD_KERNEL_PREP = 0,
+  QMHD_KERNEL_RECONSTRUCT = 1,
+  QMHD_KERNEL_FLUX = 2,
+  QMHD_KERNEL_CELL_EMF = 3,
+  QMHD_KERNEL_CORNER = 4,
+  QMHD_KERNEL_UPDATE = 5,
+  QMHD_KERNEL_GHOSTS = 6,
+  QMHD_KERNEL_REDUCE = 7,
+  QMHD_KERNEL_KINDS = 8
+};
+int qmhd_solver_set_profiling(qmhd_solver* s, int enable);
+int qmhd_solver_kernel_time(qmhd_solver* s, int kind, double* total_ms, long long* launches);
+const char* qmhd_kernel_kind_name(int kind);
+/* The solver's HIP stream (hipStream_t), on which all its work is ordered. */
+void* qmhd_solver_stream(qmhd_solver* s);
+/* Wait for all work issued by the solver (device_synchronize in the
+ * reference's compute_residual, src/physics/mhd/mhd_solver.cpp:825). */
+int qmhd_solver_synchronize(qmhd_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUASAR_MHD_H */

@@ -1,0 +1,129 @@
+"""Loader for the gfx950 library (quasar_amd/lib/libquasar_mhd.so) and the
+ctypes prototypes of its C ABI (include/quasar_mhd.h).
+
+There is no CPU fallback: if the library is missing the import fails loudly."""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libquasar_mhd.so"
+
+D = C.c_double
+DP = C.POINTER(C.c_double)
+I32P = C.POINTER(C.c_int32)
+VP = C.c_void_p
+
+
+class QGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("ng", C.c_int32), ("reserved", C.c_int32),
+                ("lx", D), ("ly", D), ("origin_x", D), ("origin_y", D), ("cell_dx", D),
+                ("cell_dy", D)]
+
+
+class QIcSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32),
+                ("gamma", D), ("magnetic_scale", D), ("interface_x", D),
+                ("left", D * 8), ("right", D * 8),
+                ("rho", D), ("pressure", D), ("b0", D), ("total_b0", D), ("amplitude", D),
+                ("wavenumber", C.c_int64), ("magnetic_velocity_scale", D),
+                ("b_uniform", D * 3), ("center", D * 2),
+                ("r_in", D), ("r0", D), ("r1", D), ("rho_in", D), ("rho_out", D),
+                ("p_in", D), ("p_out", D), ("p_core", D), ("p_ambient", D),
+                ("rho_ambient", D), ("u0", D),
+                ("ot_rho", D), ("ot_p", D), ("noise_amplitude", D), ("noise_seed", C.c_uint64)]
+
+
+Name32 = C.c_char * 32
+
+
+class QSolverConfig(C.Structure):
+    _fields_ = [("grid", QGrid), ("gamma", D), ("cfl", D), ("rho_floor", D), ("p_floor", D),
+                ("reconstruction", Name32), ("riemann", Name32), ("integrator", Name32),
+                ("ct", Name32), ("positivity", Name32), ("geometry", Name32),
+                ("fluid", Name32 * 4), ("field", Name32 * 4)]
+
+
+EXPORTS = [
+    "qmhd_last_error", "qmhd_version", "qmhd_grid_storage", "qmhd_scratch_doubles",
+    "qmhd_collocate_face_to_cell", "qmhd_reconstruct", "qmhd_hlld_flux", "qmhd_flux_difference",
+    "qmhd_ct_emf", "qmhd_emf_curl_rate", "qmhd_rk_stage", "qmhd_fill_ghosts",
+    "qmhd_cfl_max_rate", "qmhd_admissible_fraction", "qmhd_divb_linf", "qmhd_divb_relative_linf",
+    "qmhd_conserved_cell_sums", "qmhd_seed_initial_state",
+    "qmhd_solver_create", "qmhd_solver_destroy", "qmhd_solver_grid", "qmhd_solver_seed_state",
+    "qmhd_solver_state_to_host", "qmhd_solver_cfl_limit", "qmhd_solver_step",
+    "qmhd_solver_step_unchecked", "qmhd_solver_advance", "qmhd_solver_divergence_b_max",
+    "qmhd_solver_conserved_cell_totals", "qmhd_solver_last_positivity_substeps",
+    "qmhd_solver_run", "qmhd_solver_state_plane", "qmhd_solver_set_profiling",
+    "qmhd_solver_kernel_time", "qmhd_kernel_kind_name", "qmhd_solver_stream",
+    "qmhd_solver_synchronize",
+]
+
+KERNEL_KINDS = ["prep", "reconstruct", "flux", "cell_emf", "corner", "update", "ghosts", "reduce"]
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the library once; raise ImportError when it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    L = C.CDLL(str(LIB_PATH))
+    L.qmhd_last_error.restype = C.c_char_p
+    L.qmhd_version.restype = C.c_char_p
+    L.qmhd_grid_storage.restype = C.c_size_t
+    L.qmhd_grid_storage.argtypes = [C.POINTER(QGrid)]
+    L.qmhd_scratch_doubles.restype = C.c_size_t
+    L.qmhd_scratch_doubles.argtypes = [C.POINTER(QGrid)]
+    L.qmhd_collocate_face_to_cell.argtypes = [C.POINTER(QGrid), C.c_int, C.c_int, VP, VP, VP]
+    L.qmhd_reconstruct.argtypes = [C.POINTER(QGrid), C.c_int, C.c_int, I32P, D, VP, VP, VP, VP]
+    L.qmhd_hlld_flux.argtypes = [C.POINTER(QGrid), C.c_int, C.c_int, C.c_int, I32P, D, VP, VP, VP, VP]
+    L.qmhd_flux_difference.argtypes = [C.POINTER(QGrid), C.c_int, VP, VP, VP]
+    L.qmhd_ct_emf.argtypes = [C.POINTER(QGrid), C.c_int, I32P, D, VP, VP, VP, VP, VP, VP, VP, VP]
+    L.qmhd_emf_curl_rate.argtypes = [C.POINTER(QGrid), VP, VP, VP, VP]
+    L.qmhd_rk_stage.argtypes = [C.POINTER(QGrid), D, D, D, VP, VP, VP, VP, VP]
+    L.qmhd_fill_ghosts.argtypes = [C.POINTER(QGrid), I32P, I32P, VP, VP]
+    L.qmhd_cfl_max_rate.argtypes = [C.POINTER(QGrid), C.c_int, I32P, D, VP, VP, DP, VP, VP]
+    L.qmhd_admissible_fraction.argtypes = [C.POINTER(QGrid), VP, VP, C.c_int, DP, VP, VP]
+    L.qmhd_divb_linf.argtypes = [C.POINTER(QGrid), VP, DP, VP, VP]
+    L.qmhd_divb_relative_linf.argtypes = [C.POINTER(QGrid), VP, C.c_int, DP, VP, VP]
+    L.qmhd_conserved_cell_sums.argtypes = [C.POINTER(QGrid), VP, DP, DP, VP, VP]
+    L.qmhd_seed_initial_state.argtypes = [C.POINTER(QGrid), C.POINTER(QIcSpec), VP, VP, VP]
+    L.qmhd_solver_create.argtypes = [C.POINTER(QSolverConfig), C.POINTER(VP)]
+    L.qmhd_solver_destroy.argtypes = [VP]
+    L.qmhd_solver_destroy.restype = None
+    L.qmhd_solver_grid.argtypes = [VP, C.POINTER(QGrid)]
+    L.qmhd_solver_seed_state.argtypes = [VP, C.c_char_p, DP, C.c_size_t]
+    L.qmhd_solver_state_to_host.argtypes = [VP, C.c_char_p, DP, C.c_size_t]
+    L.qmhd_solver_cfl_limit.argtypes = [VP, DP]
+    L.qmhd_solver_step.argtypes = [VP, D]
+    L.qmhd_solver_step_unchecked.argtypes = [VP, D]
+    L.qmhd_solver_advance.argtypes = [VP, D, D]
+    L.qmhd_solver_divergence_b_max.argtypes = [VP, DP]
+    L.qmhd_solver_conserved_cell_totals.argtypes = [VP, DP, DP]
+    L.qmhd_solver_last_positivity_substeps.argtypes = [VP]
+    L.qmhd_solver_run.argtypes = [VP, C.c_int, D, C.POINTER(C.c_int), DP]
+    L.qmhd_solver_state_plane.argtypes = [VP, C.c_int, C.POINTER(DP)]
+    L.qmhd_solver_set_profiling.argtypes = [VP, C.c_int]
+    L.qmhd_solver_kernel_time.argtypes = [VP, C.c_int, DP, C.POINTER(C.c_longlong)]
+    L.qmhd_kernel_kind_name.argtypes = [C.c_int]
+    L.qmhd_kernel_kind_name.restype = C.c_char_p
+    L.qmhd_solver_stream.argtypes = [VP]
+    L.qmhd_solver_stream.restype = VP
+    L.qmhd_solver_synchronize.argtypes = [VP]
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    """Translate a C ABI status into the reference's Python exception types."""
+    if status == 0:
+        return
+    msg = load().qmhd_last_error().decode()
+    if status == 1:
+        raise ValueError(msg)
+    raise RuntimeError(msg)

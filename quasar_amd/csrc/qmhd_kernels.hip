@@ -1,0 +1,1617 @@
+// HIP kernels (gfx950) for one SSP-RK3 stage of the 2D constrained-transport
+// ideal-MHD step: staggered-field collocation with the admissibility and
+// div(B) preflights, characteristic MP5/MP7 (MUSCL, piecewise-constant)
+// reconstruction, transverse-Gauss HLLD face fluxes with the CFL face rate,
+// the cell-average E_z tensor quadrature (LDS tile), the upwind CT corner EMF,
+// the fused flux-difference + EMF-curl + Shu-Osher update, ghost fills and the
+// analytic initial conditions.
+//
+// Storage: every plane is a padded row-major array, index (i+ng) + pitch*(j+ng),
+// pitch = nx + 2 ng, identical to the reference's Grid2D so seeds and
+// readbacks are byte compatible.  State planes are FP64; interface states and
+// fluxes are of the kernel's arithmetic type T.
+
+#include "qmhd_kernels.hpp"
+
+#include <cstdint>
+
+namespace qmhd {
+
+__constant__ QuadRows c_rows;
+
+// ---------------------------------------------------------------------------
+// Small helpers.
+
+__device__ __forceinline__ std::size_t idx(const GridD& g, int i, int j) {
+  return static_cast<std::size_t>(i + g.ng) + static_cast<std::size_t>(g.pitch) * static_cast<std::size_t>(j + g.ng);
+}
+
+// Periodic wrap or clamp of a face/cell coordinate (mode 4 keeps exchanged guards).
+__device__ __forceinline__ int face_index(int q, int n, int lo_mode, int hi_mode) {
+  if (lo_mode == 0 && hi_mode == 0) {
+    int w = q % n;
+    return w < 0 ? w + n : w;
+  }
+  if (q < 0) return lo_mode == 4 ? q : 0;
+  if (q > n) return hi_mode == 4 ? q : n;
+  return q;
+}
+
+// Non-negative doubles order like their bit patterns: min/max through
+// 64-bit integer atomics is exact and order independent.
+__device__ __forceinline__ void atomic_min_nonneg(unsigned long long* slot, double v) {
+  atomicMin(slot, static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* slot, double v) {
+  atomicMax(slot, static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+// Wave-level reductions (64 lanes) before one atomic per wave.
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ bool lane_leader() { return (threadIdx.x + blockDim.x * threadIdx.y) % 64 == 0; }
+
+// ---------------------------------------------------------------------------
+// Face -> cell collocation of the staggered normal field.
+
+__device__ __forceinline__ int colloc_width(int extent, int ng) {
+  const int req = ng >= 4 ? 8 : ng >= 3 ? 6 : ng >= 2 ? 4 : 2;
+  if (req >= 8 && extent >= 8) return 8;
+  if (req >= 6 && extent >= 6) return 6;
+  if (req >= 4 && extent >= 4) return 4;
+  return extent >= 2 ? 2 : 1;
+}
+
+// Sample of the face array along `axis` at coordinate q on line `fixed`.
+__device__ __forceinline__ double axis_face(const GridD& g, const double* f, int axis, int q, int fixed) {
+  return axis == 0 ? f[idx(g, q, fixed)] : f[idx(g, fixed, q)];
+}
+
+// High-order cell average (8/6/4/2 taps, one-sided rows in the outer ghosts).
+__device__ double face_to_cell(const GridD& g, const double* f, int axis, int cell, int fixed) {
+  const int n = axis == 0 ? g.nx : g.ny;
+  const int width = colloc_width(n + 2 * g.ng, g.ng);
+  const int lo = -g.ng, hi = n + g.ng - 1;
+  if (width == 1) return axis_face(g, f, axis, cell, fixed);
+  const int start_c = cell - (width / 2 - 1);
+  if (start_c >= lo && start_c + width - 1 <= hi) {
+    if (width == 8) {
+      const double s = -191.0 * axis_face(g, f, axis, start_c, fixed) +
+                       1879.0 * axis_face(g, f, axis, start_c + 1, fixed) -
+                       9531.0 * axis_face(g, f, axis, start_c + 2, fixed) +
+                       68323.0 * axis_face(g, f, axis, start_c + 3, fixed) +
+                       68323.0 * axis_face(g, f, axis, start_c + 4, fixed) -
+                       9531.0 * axis_face(g, f, axis, start_c + 5, fixed) +
+                       1879.0 * axis_face(g, f, axis, start_c + 6, fixed) -
+                       191.0 * axis_face(g, f, axis, start_c + 7, fixed);
+      return s / 120960.0;
+    }
+    if (width == 6) {
+      const double s = 11.0 * axis_face(g, f, axis, start_c, fixed) -
+                       93.0 * axis_face(g, f, axis, start_c + 1, fixed) +
+                       802.0 * axis_face(g, f, axis, start_c + 2, fixed) +
+                       802.0 * axis_face(g, f, axis, start_c + 3, fixed) -
+                       93.0 * axis_face(g, f, axis, start_c + 4, fixed) +
+                       11.0 * axis_face(g, f, axis, start_c + 5, fixed);
+      return s / 1440.0;
+    }
+    if (width == 4) {
+      const double s = -axis_face(g, f, axis, start_c, fixed) + 13.0 * axis_face(g, f, axis, start_c + 1, fixed) +
+                       13.0 * axis_face(g, f, axis, start_c + 2, fixed) - axis_face(g, f, axis, start_c + 3, fixed);
+      return s / 24.0;
+    }
+    const double a = axis_face(g, f, axis, start_c, fixed), b = axis_face(g, f, axis, start_c + 1, fixed);
+    return a == b ? a : (a + b) / 2.0;
+  }
+  int start = start_c < lo ? lo : start_c;
+  if (start + width - 1 > hi) start = hi - width + 1;
+  const int wi = width == 8 ? 0 : width == 6 ? 1 : width == 4 ? 2 : 3;
+  const double* row = c_rows.one_sided[wi][cell - start];
+  double s = 0.0;
+  for (int k = 0; k < width; ++k) s += row[k] * axis_face(g, f, axis, start + k, fixed);
+  return s;
+}
+
+// First-order (two bounding faces) cell average used by the low-order retry.
+__device__ __forceinline__ double face_to_cell_lo(const GridD& g, const double* f, int axis, int cell, int fixed) {
+  const int n = axis == 0 ? g.nx : g.ny;
+  const double a = axis_face(g, f, axis, cell, fixed);
+  if (cell >= n + g.ng - 1) return a;
+  const double b = axis_face(g, f, axis, cell + 1, fixed);
+  return a == b ? a : (a + b) / 2.0;
+}
+
+// Internal energy q = E - K - M of a convex combination (the admissible set
+// rho > 0, q > 0 is convex).
+__device__ __forceinline__ double convex_q(const St<double>& a, const St<double>& b, double th) {
+  const double om = 1.0 - th;
+  const double rho = om * a.rho + th * b.rho;
+  if (!(rho > 0.0) || !fin(rho)) return -Lim<double>::inf();
+  const double mx = om * a.mx + th * b.mx, my = om * a.my + th * b.my, mz = om * a.mz + th * b.mz;
+  const double e = om * a.en + th * b.en;
+  const double bx = om * a.bx + th * b.bx, by = om * a.by + th * b.by, bz = om * a.bz + th * b.bz;
+  const double q = (e - half_sq(mx, my, mz) / rho) - half_sq(bx, by, bz);
+  return fin(q) ? q : -Lim<double>::inf();
+}
+
+// Largest theta in [0,1] keeping rho > 0 and q > 0 on the segment a -> b.
+__device__ double admissible_theta(const St<double>& a, const St<double>& b) {
+  const double qa = convex_q(a, b, 0.0);
+  if (!(a.rho > 0.0) || !(qa > 0.0) || !fin(a.rho) || !fin(qa)) return 0.0;
+  if (!fin(b.rho) || !fin(b.mx) || !fin(b.my) || !fin(b.mz) || !fin(b.en) || !fin(b.bx) || !fin(b.by) || !fin(b.bz))
+    return 0.0;
+  double hi = 1.0;
+  if (b.rho <= 0.0) {
+    const double d = a.rho - b.rho;
+    hi = d > 0.0 ? a.rho / d : 0.0;
+    hi = hi < 0.0 ? 0.0 : (hi > 1.0 ? 1.0 : hi);
+  }
+  if (convex_q(a, b, hi) > 0.0) return hi;
+  double lo = 0.0;
+  for (int it = 0; it < 56; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (convex_q(a, b, mid) > 0.0)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ St<double> cell_state(const GridD& g, const StateView& u, std::size_t k, double bx, double by) {
+  return St<double>{u.c[0][k], u.c[1][k], u.c[2][k], u.c[3][k], u.c[4][k], bx, by, u.c[7][k]};
+}
+
+// ulp of a binary64 value as used by the solenoidality roundoff budget.
+__device__ __forceinline__ double ulp_of(double v) {
+  if (!fin(v)) return Lim<double>::inf();
+  const double m = __builtin_fabs(v);
+  if (m < Lim<double>::tiny) return 4.9406564584124654e-324;
+  int e = 0;
+  (void)frexp(m, &e);
+  return ldexp(0.5, e - 52);
+}
+
+// Prepare kernel: collocated Bx/By of `u` over the padded grid, and on interior
+// cells (a) the admissible fraction of u against `base` (stage check), (b) the
+// live-state admissibility under both collocations, (c) the relative div(B)
+// residual/scale maxima.  Results go to `status` through order-independent
+// atomics.
+__global__ __launch_bounds__(256) void prep_kernel(GridD g, StateView u, double* __restrict__ bxc,
+                                                   double* __restrict__ byc, StateView base,
+                                                   const double* __restrict__ base_bxc,
+                                                   const double* __restrict__ base_byc, int theta_slot,
+                                                   int theta_co, int theta_both, int live_checks,
+                                                   int solver_owned, StepStatus* __restrict__ status) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) - g.ng;
+  const int j = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y) - g.ng;
+  const bool in_pad = i < g.nx + g.ng && j < g.ny + g.ng;
+  double bxv = 0.0, byv = 0.0;
+  if (in_pad) {
+    bxv = face_to_cell(g, u.c[5], 0, i, j);
+    byv = face_to_cell(g, u.c[6], 1, j, i);
+    const std::size_t k = idx(g, i, j);
+    bxc[k] = bxv;
+    byc[k] = byv;
+  }
+  const bool interior = in_pad && i >= 0 && j >= 0 && i < g.nx && j < g.ny;
+  double th = 1.0, th_other = 1.0, live0 = 1.0, live1 = 1.0, res = 0.0, scl = 0.0;
+  if (interior) {
+    const std::size_t k = idx(g, i, j);
+    const St<double> cu = cell_state(g, u, k, bxv, byv);
+    if (theta_slot >= 0) {
+      double t0 = 1.0, t1 = 1.0;
+      if (theta_co == 0 || theta_both) {
+        const St<double> cb = cell_state(g, base, k, base_bxc[k], base_byc[k]);
+        t0 = admissible_theta(cb, cu);
+      }
+      if (theta_co == 1 || theta_both) {
+        const St<double> cb1 = cell_state(g, base, k, face_to_cell_lo(g, base.c[5], 0, i, j),
+                                          face_to_cell_lo(g, base.c[6], 1, j, i));
+        const St<double> cu1 = cell_state(g, u, k, face_to_cell_lo(g, u.c[5], 0, i, j),
+                                          face_to_cell_lo(g, u.c[6], 1, j, i));
+        t1 = admissible_theta(cb1, cu1);
+      }
+      th = theta_co == 0 ? t0 : t1;
+      th_other = theta_co == 0 ? t1 : t0;
+    }
+    if (live_checks) {
+      live0 = admissible_theta(cu, cu);
+      const St<double> cu1 = cell_state(g, u, k, face_to_cell_lo(g, u.c[5], 0, i, j),
+                                        face_to_cell_lo(g, u.c[6], 1, j, i));
+      live1 = admissible_theta(cu1, cu1);
+      const double bxl = u.c[5][k], bxh = u.c[5][idx(g, i + 1, j)];
+      const double byl = u.c[6][k], byh = u.c[6][idx(g, i, j + 1)];
+      const double radial = (bxh - bxl) / g.dx;
+      const double axial = (byh - byl) / g.dy;
+      if (!(fin(radial) && fin(axial))) {
+        res = Lim<double>::inf();
+        scl = Lim<double>::inf();
+      } else {
+        const double r = radial + axial;
+        const double s = __builtin_fabs(radial) + __builtin_fabs(axial);
+        const double ro = (ulp_of(bxh) / g.dx + ulp_of(bxl) / g.dx) + (ulp_of(byh) / g.dy + ulp_of(byl) / g.dy);
+        bool explained;
+        if (solver_owned) {
+          explained = r == 0.0 || (ro != 0.0 && __builtin_fabs(r) <= ldexp(ro, 10));
+        } else if (radial == 0.0 || axial == 0.0 || signbit(radial) == signbit(axial)) {
+          explained = r == 0.0;
+        } else {
+          explained = r == 0.0 || (__builtin_fabs(r) <= 0.5 * s && ro != 0.0 && __builtin_fabs(r) <= ldexp(ro, 10));
+        }
+        res = explained ? 0.0 : __builtin_fabs(r);
+        scl = s;
+      }
+    }
+  }
+  th = wave_min(fin(th) ? th : 0.0);
+  th_other = wave_min(fin(th_other) ? th_other : 0.0);
+  live0 = wave_min(fin(live0) ? live0 : 0.0);
+  live1 = wave_min(fin(live1) ? live1 : 0.0);
+  res = wave_max(res);
+  scl = wave_max(scl);
+  if (lane_leader()) {
+    if (theta_slot >= 0) atomic_min_nonneg(&status->theta[theta_slot], th);
+    if (theta_slot >= 0 && theta_both) atomic_min_nonneg(&status->theta[3], th_other);
+    if (live_checks) {
+      atomic_min_nonneg(&status->live_theta0, live0);
+      atomic_min_nonneg(&status->live_theta1, live1);
+      atomic_max_nonneg(&status->divb_residual, res);
+      atomic_max_nonneg(&status->divb_scale, scl);
+    }
+  }
+}
+
+// Stand-alone collocation (readback of bx/by and the launch-level ABI).
+__global__ __launch_bounds__(256) void collocate_kernel(GridD g, int axis, int co, const double* __restrict__ face,
+                                                        double* __restrict__ cell) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) - g.ng;
+  const int j = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y) - g.ng;
+  if (i >= g.nx + g.ng || j >= g.ny + g.ng) return;
+  double v;
+  if (axis == 0)
+    v = co == 1 ? face_to_cell_lo(g, face, 0, i, j) : face_to_cell(g, face, 0, i, j);
+  else
+    v = co == 1 ? face_to_cell_lo(g, face, 1, j, i) : face_to_cell(g, face, 1, j, i);
+  cell[idx(g, i, j)] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction.
+
+template <class T>
+__device__ __forceinline__ St<T> load_cell(const GridD& g, const StateView& u, const double* bxc,
+                                           const double* byc, int i, int j) {
+  const std::size_t k = idx(g, i, j);
+  return St<T>{T(u.c[0][k]), T(u.c[1][k]), T(u.c[2][k]), T(u.c[3][k]),
+               T(u.c[4][k]), T(bxc[k]),    T(byc[k]),    T(u.c[7][k])};
+}
+
+template <class T>
+__device__ __forceinline__ St<T> load_cell_lo(const GridD& g, const StateView& u, int i, int j) {
+  const std::size_t k = idx(g, i, j);
+  return St<T>{T(u.c[0][k]), T(u.c[1][k]), T(u.c[2][k]), T(u.c[3][k]), T(u.c[4][k]),
+               T(face_to_cell_lo(g, u.c[5], 0, i, j)), T(face_to_cell_lo(g, u.c[6], 1, j, i)), T(u.c[7][k])};
+}
+
+template <class T>
+__device__ __forceinline__ bool admissible_state(const St<T>& s, T gm1) {
+  if (!(fin(s.rho) && s.rho > T(0))) return false;
+  const T inv = T(1) / s.rho;
+  const T vx = s.mx * inv, vy = s.my * inv, vz = s.mz * inv;
+  const T p = pressure(s, gm1);
+  return fin(vx) && fin(vy) && fin(vz) && fin(p) && p > T(0) && fin(s.mx) && fin(s.my) && fin(s.mz) &&
+         fin(s.en) && fin(s.bx) && fin(s.by) && fin(s.bz);
+}
+
+template <class T>
+__device__ __forceinline__ void set_normal_b(St<T>& s, int dir, T v) {
+  T& bn = dir == 0 ? s.bx : s.by;
+  s.en = fma(-T(0.5) * bn, bn, fma(T(0.5) * v, v, s.en));
+  bn = v;
+}
+
+template <class T>
+__device__ __forceinline__ T limited_slope(T back, T fwd, int drop) {
+  if (drop < 0) return fwd;
+  if (drop > 0) return back;
+  return minmod2(back, fwd);
+}
+
+__device__ __forceinline__ int slope_drop(int c, int n, int lo, int hi) {
+  if (lo != 0 && lo != 4 && c <= 0) return -1;
+  if (hi != 0 && hi != 4 && c >= n - 1) return +1;
+  return 0;
+}
+
+// Second-order MUSCL-minmod in primitive variables (also the MP fallback).
+template <class T>
+__device__ __noinline__ void muscl_face(const GridD& g, int dir, int i, int j, T gamma, int side_lo, int side_hi,
+                                        const StateView& u, const double* bxc, const double* byc, St<T>& ul,
+                                        St<T>& ur) {
+  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+  const T gm1 = gamma - T(1);
+  St<T> c[4];
+  for (int s = 0; s < 4; ++s) c[s] = load_cell<T>(g, u, bxc, byc, i + (s - 2) * di, j + (s - 2) * dj);
+  T w[4][8];
+  for (int s = 0; s < 4; ++s) {
+    const T inv = T(1) / c[s].rho;
+    w[s][0] = c[s].rho;
+    w[s][1] = c[s].mx * inv;
+    w[s][2] = c[s].my * inv;
+    w[s][3] = c[s].mz * inv;
+    w[s][4] = pressure(c[s], gm1);
+    w[s][5] = c[s].bx;
+    w[s][6] = c[s].by;
+    w[s][7] = c[s].bz;
+  }
+  const int n_dir = dir == 0 ? g.nx : g.ny;
+  const int drop_m = slope_drop((dir == 0 ? i : j) - 1, n_dir, side_lo, side_hi);
+  const int drop_p = slope_drop(dir == 0 ? i : j, n_dir, side_lo, side_hi);
+  T wl[8], wr[8];
+  for (int q = 0; q < 8; ++q) {
+    wl[q] = w[1][q] + T(0.5) * limited_slope(w[1][q] - w[0][q], w[2][q] - w[1][q], drop_m);
+    wr[q] = w[2][q] - T(0.5) * limited_slope(w[2][q] - w[1][q], w[3][q] - w[2][q], drop_p);
+  }
+  auto to_cons = [&](const T* p) {
+    St<T> s;
+    s.rho = p[0];
+    s.mx = p[0] * p[1];
+    s.my = p[0] * p[2];
+    s.mz = p[0] * p[3];
+    s.bx = p[5];
+    s.by = p[6];
+    s.bz = p[7];
+    s.en = p[4] / gm1 + p[0] * half_sq(p[1], p[2], p[3]) + half_sq(p[5], p[6], p[7]);
+    return s;
+  };
+  ul = to_cons(wl);
+  ur = to_cons(wr);
+}
+
+template <class T, int ORDER>
+__device__ __forceinline__ void mp_char_face(const GridD& g, int dir, int i, int j, T gamma, const StateView& u,
+                                             const double* bxc, const double* byc, St<T>& ul, St<T>& ur) {
+  constexpr int H = ORDER == 5 ? 2 : 3;
+  constexpr int UW = 2 * H + 2;
+  constexpr int W = 2 * H + 1;
+  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+  St<T> cells[UW];
+#pragma unroll
+  for (int q = 0; q < UW; ++q) cells[q] = rot(load_cell<T>(g, u, bxc, byc, i + (q - 1 - H) * di, j + (q - 1 - H) * dj), dir);
+  St<T> ref;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) set_comp(ref, c, T(0.5) * comp(cells[H], c) + T(0.5) * comp(cells[H + 1], c));
+  Eig<T> e;
+  eig_build(ref, gamma, e);
+  // Deltas in the 7-vector (rho, mn, mt1, mt2, E, bt1, bt2).
+  T d[UW][7];
+#pragma unroll
+  for (int q = 0; q < UW; ++q) {
+    d[q][0] = cells[q].rho - ref.rho;
+    d[q][1] = cells[q].mx - ref.mx;
+    d[q][2] = cells[q].my - ref.my;
+    d[q][3] = cells[q].mz - ref.mz;
+    d[q][4] = cells[q].en - ref.en;
+    d[q][5] = cells[q].by - ref.by;
+    d[q][6] = cells[q].bz - ref.bz;
+  }
+  T aL[7], aR[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    aL[v] = T(0);
+    aR[v] = T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    T w[UW];
+#pragma unroll
+    for (int q = 0; q < UW; ++q) {
+      T acc = T(0);
+#pragma unroll
+      for (int v = 0; v < 7; ++v) acc += e.L[k][v] * d[q][v];
+      w[q] = acc;
+    }
+    // Left state: cells 0..W-1 (centre H), right state: reversed cells 1..W.
+    T sl[W], sr[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+      sl[s] = w[s];
+      sr[s] = w[W - s];
+    }
+    const T fl = ORDER == 5 ? mp5_face(sl) : mp7_face(sl);
+    const T fr = ORDER == 5 ? mp5_face(sr) : mp7_face(sr);
+    constexpr int o = ORDER == 5 ? 0 : 1;
+    const T wl = mp_limit(fl, sl[o], sl[o + 1], sl[o + 2], sl[o + 3], sl[o + 4]);
+    const T wr = mp_limit(fr, sr[o], sr[o + 1], sr[o + 2], sr[o + 3], sr[o + 4]);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      aL[v] += e.R[v][k] * wl;
+      aR[v] += e.R[v][k] * wr;
+    }
+  }
+  St<T> sL{ref.rho + aL[0], ref.mx + aL[1], ref.my + aL[2], ref.mz + aL[3], ref.en + aL[4], ref.bx + T(0), ref.by + aL[5], ref.bz + aL[6]};
+  St<T> sR{ref.rho + aR[0], ref.mx + aR[1], ref.my + aR[2], ref.mz + aR[3], ref.en + aR[4], ref.bx + T(0), ref.by + aR[5], ref.bz + aR[6]};
+  ul = rot(sL, dir);
+  ur = rot(sR, dir);
+}
+
+// One thread per face normal to `dir`: dir 0 faces i in [0,nx], j in [-ng,ny+ng);
+// dir 1 faces i in [-ng,nx+ng), j in [0,ny].  Writes L (0..7) and R (8..15)
+// interface planes and the per-face "L == R" flag used by the CT 1-D shortcut.
+template <class T, int ORDER>
+__global__ __launch_bounds__(256) void reconstruct_kernel(GridD g, int dir, T gamma, int side_lo, int side_hi,
+                                                          StateView u, const double* __restrict__ bxc,
+                                                          const double* __restrict__ byc, IfaceView<T> out,
+                                                          int* __restrict__ eq_flag) {
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lj = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = dir == 1 ? li - g.ng : li;
+  const int j = dir == 0 ? lj - g.ng : lj;
+  if (dir == 0) {
+    if (li > g.nx || lj >= g.ny + 2 * g.ng) return;
+  } else {
+    if (li >= g.nx + 2 * g.ng || lj > g.ny) return;
+  }
+  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+  const T gm1 = gamma - T(1);
+  St<T> ul, ur;
+  if (ORDER <= 1) {
+    ul = load_cell_lo<T>(g, u, i - di, j - dj);
+    ur = load_cell_lo<T>(g, u, i, j);
+  } else if (ORDER == 2) {
+    muscl_face<T>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+  } else {
+    mp_char_face<T, ORDER>(g, dir, i, j, gamma, u, bxc, byc, ul, ur);
+    if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1))
+      muscl_face<T>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+  }
+  const T bn = T((dir == 0 ? u.c[5] : u.c[6])[idx(g, i, j)]);
+  set_normal_b(ul, dir, bn);
+  set_normal_b(ur, dir, bn);
+  if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1)) {
+    if (ORDER <= 1) {
+      ul = load_cell_lo<T>(g, u, i - di, j - dj);
+      ur = load_cell_lo<T>(g, u, i, j);
+    } else {
+      ul = load_cell<T>(g, u, bxc, byc, i - di, j - dj);
+      ur = load_cell<T>(g, u, bxc, byc, i, j);
+    }
+    set_normal_b(ul, dir, bn);
+    set_normal_b(ur, dir, bn);
+  }
+  if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1)) {
+    const T nan = Lim<T>::nan();
+    ul = St<T>{nan, nan, nan, nan, nan, nan, nan, nan};
+    ur = ul;
+  }
+  const std::size_t k = idx(g, i, j);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    out.c[c][k] = comp(ul, c);
+    out.c[8 + c][k] = comp(ur, c);
+  }
+  bool eq = true;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) eq = eq && (comp(ul, c) == comp(ur, c));
+  eq_flag[k] = eq ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Face flux with transverse Gauss quadrature and the CFL face rate.
+
+template <class T>
+__device__ __forceinline__ St<T> riemann(const St<T>& L, const St<T>& R, int dir, int hll_only, T gamma) {
+  const St<T> f = hll_only ? lf_flux(rot(L, dir), rot(R, dir), gamma) : hlld_flux(rot(L, dir), rot(R, dir), gamma);
+  return rot(f, dir);
+}
+
+// |v_n| and c_fast of one face state in the direction `dir`; infinite when the
+// state is not admissible.
+template <class T>
+__device__ __forceinline__ void speed_parts(const St<T>& s, int dir, T gamma, T& vel, T& fast) {
+  const T gm1 = gamma - T(1);
+  const T p = pressure(s, gm1);
+  if (!(fin(s.rho) && s.rho > T(0) && fin(s.mx) && fin(s.my) && fin(s.mz) && fin(s.en) && fin(s.bx) &&
+        fin(s.by) && fin(s.bz) && fin(p) && p > T(0))) {
+    vel = Lim<T>::inf();
+    fast = Lim<T>::inf();
+    return;
+  }
+  const T bn = dir == 0 ? s.bx : s.by;
+  const T bt = dir == 0 ? s.by : s.bx;
+  fast = fast_speed_x(s.rho, p, bn, bt, s.bz, gamma);
+  vel = __builtin_fabs(dir == 0 ? s.mx : s.my) / s.rho;
+  if (!(fin(vel) && fin(fast))) {
+    vel = Lim<T>::inf();
+    fast = Lim<T>::inf();
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T face_speed(const St<T>& L, const St<T>& R, int dir, int order, T gamma) {
+  T vl, fl, vr, fr;
+  speed_parts(L, dir, gamma, vl, fl);
+  speed_parts(R, dir, gamma, vr, fr);
+  const T rate = order <= 1 ? fmax(vl + fl, vr + fr) : fmax(vl, vr) + fmax(fl, fr);
+  return fin(rate) ? rate : Lim<T>::inf();
+}
+
+// One thread per face in [0,nx] x [0,ny].  Writes the face flux (F_n of the
+// normal field is zero), and the face signal speed max(|v_n|)+max(c_f) over
+// the face average and every Gauss point, scaled by 0.5/spacing.
+template <class T, int ORDER>
+__global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_only, int periodic_dir, T gamma,
+                                                   IfaceView<T> in, FluxView<T> flux, double* __restrict__ rate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  int si = i, sj = j;
+  if (periodic_dir) {
+    if (dir == 0 && i == g.nx) si = 0;
+    if (dir == 1 && j == g.ny) sj = 0;
+  }
+  const std::size_t ko = idx(g, i, j);
+  const std::size_t k = idx(g, si, sj);
+  const T gm1 = gamma - T(1);
+  St<T> baseL, baseR;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    set_comp(baseL, c, in.c[c][k]);
+    set_comp(baseR, c, in.c[8 + c][k]);
+  }
+  const double spacing = dir == 0 ? g.dx : g.dy;
+  St<T> f;
+  T smax = face_speed(baseL, baseR, dir, ORDER, gamma);
+  if (ORDER != 5 && ORDER != 7) {
+    f = riemann(baseL, baseR, dir, hll_only, gamma);
+  } else {
+    constexpr int H = ORDER == 5 ? 2 : 3;
+    constexpr int NQ = ORDER == 5 ? 3 : 4;
+    const std::size_t step = dir == 0 ? static_cast<std::size_t>(g.pitch) : 1;
+    // Recover the point states at the Gauss nodes from 2H+1 face averages.
+    St<T> pl[NQ], pr[NQ];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      T s[2 * H + 1];
+#pragma unroll
+      for (int t = 0; t < 2 * H + 1; ++t) s[t] = in.c[c][k + (t - H) * step];
+      bool constant = true;
+#pragma unroll
+      for (int t = 0; t < 2 * H + 1; ++t) constant = constant && s[t] == s[H];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        T v = s[H];
+        if (!constant) {
+          v = T(0);
+#pragma unroll
+          for (int t = 0; t < 2 * H + 1; ++t)
+            v += T(ORDER == 5 ? c_rows.mp5_point[q][t] : c_rows.mp7_point[q][t]) * s[t];
+        }
+        if (c < 8)
+          set_comp(pl[q], c, v);
+        else
+          set_comp(pr[q], c - 8, v);
+      }
+    }
+    bool ok = true;
+    int first_bad = NQ;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const bool good = admissible_state(pl[q], gm1) && admissible_state(pr[q], gm1);
+      if (!good && first_bad == NQ) first_bad = q;
+      ok = ok && good;
+    }
+    // Rate: the face average plus every point before the first inadmissible one.
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      if (q < first_bad) smax = fmax(smax, face_speed(pl[q], pr[q], dir, ORDER, gamma));
+    if (!ok) {
+      f = riemann(baseL, baseR, dir, hll_only, gamma);
+    } else {
+      St<T> fq[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) fq[q] = riemann(pl[q], pr[q], dir, hll_only, gamma);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        bool constant = true;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) constant = constant && comp(fq[q], c) == comp(fq[NQ / 2], c);
+        T v = comp(fq[NQ / 2], c);
+        if (!constant) {
+          v = T(0);
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) v += T(ORDER == 5 ? c_rows.mp5_gauss[q] : c_rows.mp7_gauss[q]) * comp(fq[q], c);
+        }
+        set_comp(f, c, v);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) flux.c[c][ko] = comp(f, c);
+  const double sp = static_cast<double>(smax);
+  rate[ko] = (fin(sp) && sp >= 0.0) ? (sp * 0.5) / spacing : Lim<double>::inf();
+}
+
+// Additive cell CFL rate from the incident face rates, reduced to the maximum.
+__global__ __launch_bounds__(256) void cfl_reduce_kernel(GridD g, const double* __restrict__ rate_x,
+                                                         const double* __restrict__ rate_y,
+                                                         unsigned long long* __restrict__ max_rate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  double local = 0.0;
+  if (i < g.nx && j < g.ny) {
+    const double rx = rate_x[idx(g, i, j)] + rate_x[idx(g, i + 1, j)];
+    const double ry = rate_y[idx(g, i, j)] + rate_y[idx(g, i, j + 1)];
+    local = rx + ry;
+    local = fin(local) ? fmax(0.0, local) : Lim<double>::inf();
+  }
+  local = wave_max(local);
+  if (lane_leader()) atomic_max_nonneg(max_rate, local);
+}
+
+// ---------------------------------------------------------------------------
+// Cell-average E_z = vy Bx - vx By by tensor Gauss recovery (MP5: 3x3 points
+// from a 5x5 stencil, MP7: 4x4 from 7x7).  The block stages its cell stencil
+// (8 components) in LDS.
+
+template <class T>
+__device__ __forceinline__ T physical_ez(T rho, T mx, T my, T bx, T by) {
+  const T inv = T(1) / rho;
+  return (my * inv) * bx - (mx * inv) * by;
+}
+
+constexpr int kEmfBx = 16;
+constexpr int kEmfBy = 16;
+
+template <class T, int ORDER>
+__global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high, int phys_y_high, T gamma,
+                                                       StateView u, const double* __restrict__ bxc,
+                                                       const double* __restrict__ byc, double* __restrict__ cell_ez) {
+  constexpr int H = ORDER == 5 ? 2 : 3;
+  constexpr int NQ = ORDER == 5 ? 3 : 4;
+  constexpr int TW = 2 * H + 1;
+  constexpr int TX = kEmfBx + 2 * H;
+  constexpr int TY = kEmfBy + 2 * H;
+  __shared__ T tile[8][TY][TX];
+  const int i0 = blockIdx.x * kEmfBx, j0 = blockIdx.y * kEmfBy;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int t = tid; t < TX * TY; t += blockDim.x * blockDim.y) {
+    const int a = t % TX, b = t / TX;
+    const int ii = i0 + a - H, jj = j0 + b - H;
+    const bool inside = ii >= -g.ng && ii < g.nx + g.ng && jj >= -g.ng && jj < g.ny + g.ng;
+    const std::size_t k = inside ? idx(g, ii, jj) : 0;
+    tile[0][b][a] = inside ? T(u.c[0][k]) : T(1);
+    tile[1][b][a] = inside ? T(u.c[1][k]) : T(0);
+    tile[2][b][a] = inside ? T(u.c[2][k]) : T(0);
+    tile[3][b][a] = inside ? T(u.c[3][k]) : T(0);
+    tile[4][b][a] = inside ? T(u.c[4][k]) : T(1);
+    tile[5][b][a] = inside ? T(bxc[k]) : T(0);
+    tile[6][b][a] = inside ? T(byc[k]) : T(0);
+    tile[7][b][a] = inside ? T(u.c[7][k]) : T(0);
+  }
+  __syncthreads();
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  if ((i == g.nx && !phys_x_high) || (j == g.ny && !phys_y_high)) return;
+  const int a0 = threadIdx.x + H, b0 = threadIdx.y + H;
+  const T gm1 = gamma - T(1);
+  const T fallback = physical_ez(tile[0][b0][a0], tile[1][b0][a0], tile[2][b0][a0], tile[5][b0][a0], tile[6][b0][a0]);
+  T ez = fallback;
+  if (ORDER == 5 || ORDER == 7) {
+    // Stencil bounds inside the padded domain (always true for interior and
+    // high-closure cells when ng >= H + 1).
+    const bool in_range = i - H >= -g.ng && i + H <= g.nx + g.ng - 1 && j - H >= -g.ng && j + H <= g.ny + g.ng - 1;
+    if (in_range) {
+      T point[NQ][NQ];
+      bool ok = true;
+      for (int qx = 0; qx < NQ && ok; ++qx) {
+        T wx[TW];
+#pragma unroll
+        for (int t = 0; t < TW; ++t) wx[t] = T(ORDER == 5 ? c_rows.mp5_point[qx][t] : c_rows.mp7_point[qx][t]);
+        // rows[c][r]: x-recovery of component c on stencil row r.
+        T rows[8][TW];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+          for (int r = 0; r < TW; ++r) {
+            const T* line = &tile[c][b0 - H + r][a0 - H];
+            bool constant = true;
+#pragma unroll
+            for (int t = 0; t < TW; ++t) constant = constant && line[t] == line[H];
+            T v = line[H];
+            if (!constant) {
+              v = T(0);
+#pragma unroll
+              for (int t = 0; t < TW; ++t) v += wx[t] * line[t];
+            }
+            rows[c][r] = v;
+          }
+#pragma unroll
+        for (int qy = 0; qy < NQ; ++qy) {
+          St<T> st;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            bool constant = true;
+#pragma unroll
+            for (int r = 0; r < TW; ++r) constant = constant && rows[c][r] == rows[c][H];
+            T v = rows[c][H];
+            if (!constant) {
+              v = T(0);
+#pragma unroll
+              for (int r = 0; r < TW; ++r)
+                v += T(ORDER == 5 ? c_rows.mp5_point[qy][r] : c_rows.mp7_point[qy][r]) * rows[c][r];
+            }
+            set_comp(st, c, v);
+          }
+          const T p = pressure(st, gm1);
+          const bool adm = fin(st.rho) && st.rho > T(0) && fin(st.mx) && fin(st.my) && fin(st.mz) && fin(st.en) &&
+                           fin(st.bx) && fin(st.by) && fin(st.bz) && fin(p) && p > T(0);
+          if (!adm) {
+            ok = false;
+            break;
+          }
+          point[qy][qx] = physical_ez(st.rho, st.mx, st.my, st.bx, st.by);
+        }
+      }
+      if (ok) {
+        T row[NQ];
+#pragma unroll
+        for (int qy = 0; qy < NQ; ++qy) {
+          bool constant = true;
+#pragma unroll
+          for (int qx = 0; qx < NQ; ++qx) constant = constant && point[qy][qx] == point[qy][NQ / 2];
+          T v = point[qy][NQ / 2];
+          if (!constant) {
+            v = T(0);
+#pragma unroll
+            for (int qx = 0; qx < NQ; ++qx) v += T(ORDER == 5 ? c_rows.mp5_gauss[qx] : c_rows.mp7_gauss[qx]) * point[qy][qx];
+          }
+          row[qy] = v;
+        }
+        bool constant = true;
+#pragma unroll
+        for (int qy = 0; qy < NQ; ++qy) constant = constant && row[qy] == row[NQ / 2];
+        ez = row[NQ / 2];
+        if (!constant) {
+          ez = T(0);
+#pragma unroll
+          for (int qy = 0; qy < NQ; ++qy) ez += T(ORDER == 5 ? c_rows.mp5_gauss[qy] : c_rows.mp7_gauss[qy]) * row[qy];
+        }
+      }
+    }
+  }
+  cell_ez[idx(g, i, j)] = static_cast<double>(ez);
+}
+
+// Low-order (direct) cell E_z with the two-face collocation.
+__global__ __launch_bounds__(256) void cell_emf_lo_kernel(GridD g, int phys_x_high, int phys_y_high, int co,
+                                                          StateView u, const double* __restrict__ bxc,
+                                                          const double* __restrict__ byc, double* __restrict__ cell_ez) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  if ((i == g.nx && !phys_x_high) || (j == g.ny && !phys_y_high)) return;
+  const std::size_t k = idx(g, i, j);
+  const double bx = co == 1 ? face_to_cell_lo(g, u.c[5], 0, i, j) : bxc[k];
+  const double by = co == 1 ? face_to_cell_lo(g, u.c[6], 1, j, i) : byc[k];
+  cell_ez[k] = physical_ez(u.c[0][k], u.c[1][k], u.c[2][k], bx, by);
+}
+
+// ---------------------------------------------------------------------------
+// CT corner EMF: E_z = ex + ey - e_c, with ex/ey the face-average -> edge
+// interpolations of the directional Godunov EMFs (-F_x(By), +F_y(Bx)) and e_c
+// the cell-average -> corner tensor interpolation of the cell E_z, plus the
+// exact one-dimensional shortcuts.
+
+__device__ __forceinline__ double edge_interp(int order, const double* e) {
+  if (order >= 7) {
+    bool constant = true;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) constant = constant && e[k] == e[0];
+    if (constant) return e[0];
+    return (-3.0 * e[0] + 29.0 * e[1] - 139.0 * e[2] + 533.0 * e[3] + 533.0 * e[4] - 139.0 * e[5] + 29.0 * e[6] -
+            3.0 * e[7]) / 840.0;
+  }
+  if (order >= 5) {
+    bool constant = true;
+#pragma unroll
+    for (int k = 1; k < 6; ++k) constant = constant && e[k] == e[0];
+    if (constant) return e[0];
+    return (e[0] - 8.0 * e[1] + 37.0 * e[2] + 37.0 * e[3] - 8.0 * e[4] + e[5]) / 60.0;
+  }
+  return e[0] == e[1] ? e[0] : (e[0] + e[1]) / 2.0;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void corner_kernel(GridD g, int order, Flags4 m, FluxView<T> fx, FluxView<T> fy,
+                                                     const double* __restrict__ cell_ez, const int* __restrict__ eqx,
+                                                     const int* __restrict__ eqy, double* __restrict__ ez) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  const std::size_t k = idx(g, i, j);
+  const bool conducting = (i == 0 && (m.side[0] == 2 || m.side[0] == 3)) || (i == g.nx && m.side[1] == 2) ||
+                          (j == 0 && m.side[2] == 2) || (j == g.ny && m.side[3] == 2);
+  if (conducting) {
+    ez[k] = 0.0;
+    return;
+  }
+  const int lo = order >= 7 ? -4 : order >= 5 ? -3 : -1;
+  const int nt = order >= 7 ? 8 : order >= 5 ? 6 : 2;
+  const int rh = order >= 7 ? 3 : order >= 5 ? 2 : 0;  // transverse recovery half-width
+  auto mi = [&](int q) { return face_index(q, g.nx, m.side[0], m.side[1]); };
+  auto mj = [&](int q) { return face_index(q, g.ny, m.side[2], m.side[3]); };
+  double e[8];
+  for (int t = 0; t < nt; ++t) e[t] = -static_cast<double>(fx.c[6][idx(g, mi(i), mj(j + lo + t))]);
+  const double ex = edge_interp(order, e);
+  for (int t = 0; t < nt; ++t) e[t] = static_cast<double>(fy.c[5][idx(g, mi(i + lo + t), mj(j))]);
+  const double ey = edge_interp(order, e);
+  double line[8];
+  for (int q = 0; q < nt; ++q) {
+    for (int t = 0; t < nt; ++t) e[t] = cell_ez[idx(g, mi(i + lo + t), mj(j + lo + q))];
+    line[q] = edge_interp(order, e);
+  }
+  const double centered = edge_interp(order, line);
+  // x-only data: every y-face problem in the stencil has no jump and the cell
+  // E_z is constant along y; y-only symmetrically.
+  // Face `t` of the interpolation stencil is mapped into the face table first;
+  // its point-recovery problem then spans offsets -rh..rh about that mapped
+  // face, mapped again (periodic wrap or physical clamp).
+  auto no_jump_line = [&](const int* eq, int d) {
+    for (int t = 0; t < nt; ++t) {
+      const int fi = d == 0 ? mi(i) : mi(i + lo + t);
+      const int fj = d == 0 ? mj(j + lo + t) : mj(j);
+      for (int off = -rh; off <= rh; ++off) {
+        const int ii = d == 0 ? mi(fi) : mi(fi + off);
+        const int jj = d == 0 ? mj(fj + off) : mj(fj);
+        if (ii < -g.ng || ii >= g.nx + g.ng || jj < -g.ng || jj >= g.ny + g.ng || eq[idx(g, ii, jj)] == 0)
+          return false;
+      }
+    }
+    return true;
+  };
+  auto constant_along = [&](int axis) {
+    for (int v = 0; v < nt; ++v) {
+      const double ref = axis == 0 ? cell_ez[idx(g, mi(i + lo), mj(j + lo + v))] : cell_ez[idx(g, mi(i + lo + v), mj(j + lo))];
+      for (int w = 1; w < nt; ++w) {
+        const double c = axis == 0 ? cell_ez[idx(g, mi(i + lo + w), mj(j + lo + v))]
+                                   : cell_ez[idx(g, mi(i + lo + v), mj(j + lo + w))];
+        if (c != ref) return false;
+      }
+    }
+    return true;
+  };
+  const bool x_only = no_jump_line(eqy, 1) && constant_along(1);
+  const bool y_only = no_jump_line(eqx, 0) && constant_along(0);
+  double r;
+  if (x_only && !y_only)
+    r = ex;
+  else if (y_only && !x_only)
+    r = ey;
+  else
+    r = (ex + ey) - centered;
+  ez[k] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Fused residual + Shu-Osher stage: out = a Un + b Us + c L(U) with
+// L = -div F for the cell components and the EMF curl for the faces.  A zero
+// weight never reads its operand (rejected stages may hold NaN).
+
+__device__ __forceinline__ double rk_combine(double a, double un, double b, double us, double c, double d) {
+  const bool ua = a != 0.0, ub = b != 0.0, uc = c != 0.0;
+  if (ua) {
+    if (ub) return uc ? fma(c, d, fma(b, us, a * un)) : fma(b, us, a * un);
+    return uc ? fma(c, d, a * un) : a * un;
+  }
+  if (ub) return uc ? fma(c, d, b * us) : b * us;
+  return uc ? c * d : 0.0;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void update_kernel(GridD g, double a, double b, double cw,
+                                                     const double* __restrict__ dt_dev, StateView un, StateView us,
+                                                     FluxView<T> fx, FluxView<T> fy, const double* __restrict__ ez,
+                                                     StateOut out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  const double c = dt_dev ? cw * dt_dev[0] : cw;
+  const bool copy = a == 1.0 && b == 0.0 && c == 0.0;
+  const std::size_t k = idx(g, i, j);
+  if (i < g.nx && j < g.ny) {
+    const std::size_t kx = idx(g, i + 1, j), ky = idx(g, i, j + 1);
+    const int comps[6] = {0, 1, 2, 3, 4, 7};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int cc = comps[q];
+      if (copy) {
+        out.c[cc][k] = un.c[cc][k];
+        continue;
+      }
+      const double dx_term = (static_cast<double>(fx.c[cc][kx]) - static_cast<double>(fx.c[cc][k])) / g.dx;
+      const double dy_term = (static_cast<double>(fy.c[cc][ky]) - static_cast<double>(fy.c[cc][k])) / g.dy;
+      const double d = -dx_term - dy_term;
+      out.c[cc][k] = rk_combine(a, un.c[cc][k], b, b != 0.0 ? us.c[cc][k] : 0.0, c, d);
+    }
+  }
+  if (j < g.ny) {
+    const double d = (ez[k] - ez[idx(g, i, j + 1)]) / g.dy;
+    out.c[5][k] = copy ? un.c[5][k] : rk_combine(a, un.c[5][k], b, b != 0.0 ? us.c[5][k] : 0.0, c, d);
+  }
+  if (i < g.nx) {
+    const double d = (ez[idx(g, i + 1, j)] - ez[k]) / g.dx;
+    out.c[6][k] = copy ? un.c[6][k] : rk_combine(a, un.c[6][k], b, b != 0.0 ? us.c[6][k] : 0.0, c, d);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ghost fills: periodic / outflow / wall / axis maps per side, full padded
+// transverse range so corner blocks are closed by the later y pass.
+
+__device__ __forceinline__ void ghost_map(int n, int layer, bool lo, int mode, int& gi, int& si) {
+  gi = lo ? -layer : n - 1 + layer;
+  if (mode == 0)
+    si = lo ? n - layer : layer - 1;
+  else if (mode == 1)
+    si = lo ? 0 : n - 1;
+  else
+    si = lo ? layer - 1 : n - layer;
+}
+
+__global__ __launch_bounds__(256) void ghost_kernel(GridD g, int axis, Flags4 fluid, Flags4 field, StateOut u) {
+  // One thread per (side, layer, line) on the two sides of `axis`.
+  const int line_count = axis == 0 ? g.ny + 2 * g.ng : g.nx + 2 * g.ng;
+  const int per_side = g.ng * line_count;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= 2 * per_side) return;
+  const int side = axis * 2 + tid / per_side;
+  const int r = tid % per_side;
+  const int layer = r / line_count + 1;
+  const int line = -g.ng + r % line_count;
+  const bool lo = (side % 2) == 0;
+  const int n = axis == 0 ? g.nx : g.ny;
+  {
+    const int mode = fluid.side[side];
+    if (!(mode == 4 || (mode == 3 && side != 0))) {
+      int gi, si;
+      ghost_map(n, layer, lo, mode, gi, si);
+      const double sn = (mode == 2 || mode == 3) ? -1.0 : 1.0;
+      const std::size_t kd = axis == 0 ? idx(g, gi, line) : idx(g, line, gi);
+      const std::size_t ks = axis == 0 ? idx(g, si, line) : idx(g, line, si);
+      u.c[0][kd] = u.c[0][ks];
+      u.c[1][kd] = (axis == 0 ? sn : 1.0) * u.c[1][ks];
+      u.c[2][kd] = (axis == 0 ? 1.0 : sn) * u.c[2][ks];
+      u.c[3][kd] = (axis == 0 && mode == 3 ? -1.0 : 1.0) * u.c[3][ks];
+      u.c[4][kd] = u.c[4][ks];
+    }
+  }
+  {
+    const int mode = field.side[side];
+    if (mode == 4 || (mode == 3 && side != 0)) return;
+    int gi, si;
+    ghost_map(n, layer, lo, mode, gi, si);
+    const double sn = (mode == 2 || mode == 3) ? -1.0 : 1.0;
+    if (axis == 0) {
+      const int jj = line;
+      const std::size_t kd = idx(g, gi, jj), ks = idx(g, si, jj);
+      u.c[6][kd] = u.c[6][ks];
+      u.c[7][kd] = (mode == 3 ? -1.0 : 1.0) * u.c[7][ks];
+      double* bx = u.c[5];
+      if (mode == 2 || mode == 3) {
+        if (lo) {
+          bx[idx(g, 0, jj)] = 0.0;
+          bx[idx(g, -layer, jj)] = -bx[idx(g, layer, jj)];
+        } else if (layer == 1) {
+          bx[idx(g, g.nx, jj)] = 0.0;
+        } else {
+          bx[idx(g, g.nx + layer - 1, jj)] = -bx[idx(g, g.nx - layer + 1, jj)];
+        }
+      } else if (mode == 1 && !lo) {
+        if (layer > 1) bx[idx(g, g.nx - 1 + layer, jj)] = bx[idx(g, g.nx, jj)];
+      } else {
+        bx[kd] = sn * bx[ks];
+      }
+    } else {
+      const int ii = line;
+      const std::size_t kd = idx(g, ii, gi), ks = idx(g, ii, si);
+      u.c[5][kd] = u.c[5][ks];
+      u.c[7][kd] = u.c[7][ks];
+      double* by = u.c[6];
+      if (mode == 2) {
+        if (lo) {
+          by[idx(g, ii, 0)] = 0.0;
+          by[idx(g, ii, -layer)] = -by[idx(g, ii, layer)];
+        } else if (layer == 1) {
+          by[idx(g, ii, g.ny)] = 0.0;
+        } else {
+          by[idx(g, ii, g.ny + layer - 1)] = -by[idx(g, ii, g.ny - layer + 1)];
+        }
+      } else if (mode == 1 && !lo) {
+        if (layer > 1) by[idx(g, ii, g.ny - 1 + layer)] = by[idx(g, ii, g.ny)];
+      } else {
+        by[kd] = sn * by[ks];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dt from the reduced rate (device side, for the auto-dt stepping loop).
+__global__ void dt_kernel(const unsigned long long* __restrict__ max_rate_bits, double effective_cfl, double dx,
+                          double dy, double* __restrict__ dt_out, double* __restrict__ t_accum) {
+  const double rate = __longlong_as_double(static_cast<long long>(max_rate_bits[0]));
+  double dt;
+  if (!(rate > 0.0)) {
+    const double hmin = fmin(dx, dy), hmax = fmax(dx, dy);
+    dt = effective_cfl * (hmin / (1.0 + hmin / hmax));
+  } else {
+    dt = effective_cfl / rate;
+  }
+  dt_out[0] = dt;
+  if (t_accum) t_accum[0] += dt;
+}
+
+// ---------------------------------------------------------------------------
+// Analytic initial conditions (two passes: primitives -> conserved with the raw
+// magnetic energy, then energy re-formed with the collocated field).
+
+__device__ __forceinline__ std::uint64_t mulhilo32(std::uint32_t a, std::uint32_t b, std::uint32_t& hi) {
+  const std::uint64_t p = static_cast<std::uint64_t>(a) * b;
+  hi = static_cast<std::uint32_t>(p >> 32);
+  return p;
+}
+
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw 2011): uniform on [-1, 1) from the
+// 53 high bits of the first two output words.
+__device__ double philox_pm1(std::uint64_t seed, std::uint32_t comp_id, std::uint64_t cell) {
+  std::uint32_t c0 = static_cast<std::uint32_t>(cell), c1 = static_cast<std::uint32_t>(cell >> 32), c2 = comp_id, c3 = 0;
+  std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    std::uint32_t hi0, hi1;
+    const std::uint32_t lo0 = static_cast<std::uint32_t>(mulhilo32(0xD2511F53u, c0, hi0));
+    const std::uint32_t lo1 = static_cast<std::uint32_t>(mulhilo32(0xCD9E8D57u, c2, hi1));
+    const std::uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  const std::uint64_t bits = (static_cast<std::uint64_t>(c1) << 32) | c0;
+  return 2.0 * (static_cast<double>(bits >> 11) * 0x1.0p-53) - 1.0;
+}
+
+__global__ __launch_bounds__(256) void seed_kernel(GridD g, IcSpecD s, StateOut out, double* __restrict__ raw_mag,
+                                                   int* __restrict__ status) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) - g.ng;
+  const int j = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y) - g.ng;
+  if (i >= g.nx + g.ng || j >= g.ny + g.ng) return;
+  const double pi = 3.141592653589793238462643383279502884;
+  const double xc = fma(static_cast<double>(i) + 0.5, g.dx, g.ox);
+  const double yc = fma(static_cast<double>(j) + 0.5, g.dy, g.oy);
+  double rho = 1, p = 1, vx = 0, vy = 0, vz = 0, bx = 0, by = 0, bz = 0, eoff = 0;
+  switch (s.kind) {
+    case 0: {
+      const double* side = xc < s.interface_x ? s.left : s.right;
+      rho = side[0]; p = side[1]; vx = side[2]; vy = side[3]; vz = side[4]; bx = side[5]; by = side[6]; bz = side[7];
+      break;
+    }
+    case 1: {
+      const double n = static_cast<double>(s.wavenumber);
+      const double kw = 2.0 * pi * n / g.lx;
+      const double phase = kw * (xc - g.ox);
+      const double q = n / static_cast<double>(g.nx);
+      const double fv = q == 0.0 ? 1.0 : sin(pi * q) / (pi * q);
+      const double isr = copysign(s.magnetic_velocity_scale, s.total_b0) / sqrt(s.rho);
+      rho = s.rho; p = s.pressure;
+      vy = -s.amplitude * isr * fv * sin(phase);
+      vz = -s.amplitude * isr * fv * cos(phase);
+      bx = s.b0; by = s.amplitude * fv * sin(phase); bz = s.amplitude * fv * cos(phase);
+      const double sa = s.amplitude * s.magnetic_velocity_scale;
+      eoff = sa * sa * (1.0 - fv * fv);
+      break;
+    }
+    case 2: {
+      const double xi = (xc - g.ox) / g.lx, eta = (yc - g.oy) / g.ly;
+      const double two_pi = 2.0 * pi;
+      const double b0 = s.b_uniform[0];
+      rho = s.ot_rho != 0.0 ? s.ot_rho : s.gamma * s.gamma;
+      p = s.ot_p != 0.0 ? s.ot_p : s.gamma;
+      vx = -sin(two_pi * eta);
+      vy = sin(two_pi * xi);
+      bx = -b0 * sin(two_pi * eta);
+      by = b0 * sin(2.0 * two_pi * xi);
+      break;
+    }
+    case 3: {
+      const double r = hypot(xc - s.center[0], yc - s.center[1]);
+      rho = s.rho_ambient; p = r < s.r_in ? s.p_core : s.p_ambient;
+      bx = s.b_uniform[0]; by = s.b_uniform[1]; bz = s.b_uniform[2];
+      break;
+    }
+    case 4: {
+      const double rx = xc - s.center[0], ry = yc - s.center[1];
+      const double r = hypot(rx, ry);
+      double f = (s.r1 - r) / (s.r1 - s.r0);
+      f = fmin(fmax(f, 0.0), 1.0);
+      const bool inside = r <= s.r0;
+      if (inside) f = 1.0;
+      if (r >= s.r1) f = 0.0;
+      double speed = 0.0;
+      if (inside)
+        speed = s.u0 / s.r0;
+      else if (r < s.r1)
+        speed = f * s.u0 / r;
+      rho = s.rho_out + f * (s.rho_in - s.rho_out);
+      p = s.p_ambient;
+      vx = -speed * ry;
+      vy = speed * rx;
+      bx = s.b_uniform[0]; by = s.b_uniform[1]; bz = s.b_uniform[2];
+      break;
+    }
+    default:
+      atomicOr(status, 1);
+      return;
+  }
+  if (s.noise_amplitude != 0.0) {
+    int wi = i % g.nx;
+    if (wi < 0) wi += g.nx;
+    int wj = j % g.ny;
+    if (wj < 0) wj += g.ny;
+    const std::uint64_t cell = static_cast<std::uint64_t>(wj) * static_cast<std::uint64_t>(g.nx) + static_cast<std::uint64_t>(wi);
+    rho = rho * (1.0 + s.noise_amplitude * philox_pm1(s.noise_seed, 0, cell));
+    vx = vx + s.noise_amplitude * philox_pm1(s.noise_seed, 1, cell);
+    vy = vy + s.noise_amplitude * philox_pm1(s.noise_seed, 2, cell);
+  }
+  if (!(fin(rho) && rho > 0.0)) {
+    atomicOr(status, 2);
+    return;
+  }
+  if (!(fin(p) && p > 0.0)) {
+    atomicOr(status, 4);
+    return;
+  }
+  // Scaled forms of rho|v|^2/2 and |B|^2/2 as the reference seeds them.
+  auto kin_v = [](double r, double x, double y, double z) {
+    const double sc = fmax(fabs(x), fmax(fabs(y), fabs(z)));
+    if (sc == 0.0) return 0.0;
+    const double xs = x / sc, ys = y / sc, zs = z / sc;
+    const double q = sqrt(r) * sc;
+    return (0.5 * q) * (q * (xs * xs + ys * ys + zs * zs));
+  };
+  auto hsq = [](double x, double y, double z) {
+    const double sc = fmax(fabs(x), fmax(fabs(y), fabs(z)));
+    if (sc == 0.0) return 0.0;
+    const double xs = x / sc, ys = y / sc, zs = z / sc;
+    return (0.5 * sc) * (sc * (xs * xs + ys * ys + zs * zs));
+  };
+  const double kin = kin_v(rho, vx, vy, vz);
+  const double mag = hsq(bx, by, bz);
+  const std::size_t k = idx(g, i, j);
+  out.c[0][k] = rho;
+  out.c[1][k] = rho * vx;
+  out.c[2][k] = rho * vy;
+  out.c[3][k] = rho * vz;
+  out.c[4][k] = p / (s.gamma - 1.0) + kin + mag + eoff;
+  out.c[5][k] = s.magnetic_scale * bx;
+  out.c[6][k] = s.magnetic_scale * by;
+  out.c[7][k] = s.magnetic_scale * bz;
+  raw_mag[k] = mag;
+}
+
+__global__ __launch_bounds__(256) void recollocate_kernel(GridD g, double gamma, StateOut out,
+                                                          const double* __restrict__ raw_mag, int* __restrict__ status) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) - g.ng;
+  const int j = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y) - g.ng;
+  if (i >= g.nx + g.ng || j >= g.ny + g.ng) return;
+  const std::size_t k = idx(g, i, j);
+  const double bxc = face_to_cell(g, out.c[5], 0, i, j);
+  const double byc = face_to_cell(g, out.c[6], 1, j, i);
+  const double bz = out.c[7][k];
+  const double sc = fmax(fabs(bxc), fmax(fabs(byc), fabs(bz)));
+  double mag = 0.0;
+  if (sc != 0.0) {
+    const double xs = bxc / sc, ys = byc / sc, zs = bz / sc;
+    mag = (0.5 * sc) * (sc * (xs * xs + ys * ys + zs * zs));
+  }
+  const double e = (out.c[4][k] - raw_mag[k]) + mag;
+  out.c[4][k] = e;
+  const St<double> st{out.c[0][k], out.c[1][k], out.c[2][k], out.c[3][k], e, bxc, byc, bz};
+  if (!(pressure(st, gamma - 1.0) > 0.0)) atomicOr(status, 4);
+}
+
+// ---------------------------------------------------------------------------
+// Launch-level helpers mirroring the reference ABI.
+
+__global__ __launch_bounds__(256) void rk_combine_kernel(GridD g, double a, double b, double c, StateView un,
+                                                         StateView us, StateView dudt, StateOut out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  const bool copy = a == 1.0 && b == 0.0 && c == 0.0;
+  const std::size_t k = idx(g, i, j);
+  for (int cc = 0; cc < 8; ++cc) {
+    const bool active = cc == 5 ? j < g.ny : cc == 6 ? i < g.nx : (i < g.nx && j < g.ny);
+    if (!active) continue;
+    out.c[cc][k] = copy ? un.c[cc][k]
+                        : rk_combine(a, un.c[cc][k], b, b != 0.0 ? us.c[cc][k] : 0.0, c,
+                                     c != 0.0 ? dudt.c[cc][k] : 0.0);
+  }
+}
+
+__global__ __launch_bounds__(256) void divb_linf_kernel(GridD g, StateView u, unsigned long long* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  double local = 0.0;
+  if (i < g.nx && j < g.ny) {
+    const double d = (u.c[5][idx(g, i + 1, j)] - u.c[5][idx(g, i, j)]) / g.dx +
+                     (u.c[6][idx(g, i, j + 1)] - u.c[6][idx(g, i, j)]) / g.dy;
+    local = fin(d) ? __builtin_fabs(d) : Lim<double>::inf();
+  }
+  local = wave_max(local);
+  if (lane_leader()) atomic_max_nonneg(out, local);
+}
+
+__global__ __launch_bounds__(256) void theta_kernel(GridD g, StateView base, StateView cand, int co,
+                                                    unsigned long long* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  double th = 1.0;
+  if (i < g.nx && j < g.ny) {
+    const std::size_t k = idx(g, i, j);
+    const double bxa = co == 1 ? face_to_cell_lo(g, base.c[5], 0, i, j) : face_to_cell(g, base.c[5], 0, i, j);
+    const double bya = co == 1 ? face_to_cell_lo(g, base.c[6], 1, j, i) : face_to_cell(g, base.c[6], 1, j, i);
+    const double bxb = co == 1 ? face_to_cell_lo(g, cand.c[5], 0, i, j) : face_to_cell(g, cand.c[5], 0, i, j);
+    const double byb = co == 1 ? face_to_cell_lo(g, cand.c[6], 1, j, i) : face_to_cell(g, cand.c[6], 1, j, i);
+    th = admissible_theta(cell_state(g, base, k, bxa, bya), cell_state(g, cand, k, bxb, byb));
+    th = fin(th) ? th : 0.0;
+  }
+  th = wave_min(th);
+  if (lane_leader()) atomic_min_nonneg(out, th);
+}
+
+// Per-block compensated (two-sum) partial sums of mass and energy; the host
+// folds the 2x2 partials per block exactly.
+__global__ __launch_bounds__(256) void cell_sums_kernel(GridD g, StateView u, double* __restrict__ partials) {
+  __shared__ double s_val[4][256];
+  const int tid = threadIdx.x;
+  double sm = 0.0, cm = 0.0, se = 0.0, ce = 0.0;
+  const long long cells = static_cast<long long>(g.nx) * g.ny;
+  for (long long c = blockIdx.x * 256LL + tid; c < cells; c += static_cast<long long>(gridDim.x) * 256) {
+    const int i = static_cast<int>(c % g.nx), j = static_cast<int>(c / g.nx);
+    const std::size_t k = idx(g, i, j);
+    {
+      const double x = u.c[0][k];
+      const double t = sm + x;
+      const double bp = t - sm;
+      cm += (sm - (t - bp)) + (x - bp);
+      sm = t;
+    }
+    {
+      const double x = u.c[4][k];
+      const double t = se + x;
+      const double bp = t - se;
+      ce += (se - (t - bp)) + (x - bp);
+      se = t;
+    }
+  }
+  s_val[0][tid] = sm;
+  s_val[1][tid] = cm;
+  s_val[2][tid] = se;
+  s_val[3][tid] = ce;
+  __syncthreads();
+  if (tid < 4) {
+    // Exact-enough per-block fold of 256 (sum, correction) pairs with two-sum.
+    double s = 0.0, comp_acc = 0.0;
+    for (int t = 0; t < 256; ++t) {
+      const double x = s_val[tid][t];
+      const double r = s + x;
+      const double bp = r - s;
+      comp_acc += (s - (r - bp)) + (x - bp);
+      s = r;
+    }
+    partials[(blockIdx.x * 4 + tid) * 2 + 0] = s;
+    partials[(blockIdx.x * 4 + tid) * 2 + 1] = comp_acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void eq_flag_kernel(GridD g, int dir, IfaceView<double> in, int* __restrict__ eq) {
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lj = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = dir == 1 ? li - g.ng : li;
+  const int j = dir == 0 ? lj - g.ng : lj;
+  if (dir == 0) {
+    if (li > g.nx || lj >= g.ny + 2 * g.ng) return;
+  } else {
+    if (li >= g.nx + 2 * g.ng || lj > g.ny) return;
+  }
+  const std::size_t k = idx(g, i, j);
+  bool e = true;
+  for (int c = 0; c < 8; ++c) e = e && (in.c[c][k] == in.c[8 + c][k]);
+  eq[k] = e ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void flux_diff_kernel(GridD g, int dir, FluxView<double> f, StateOut d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const std::size_t k = idx(g, i, j);
+  const std::size_t kp = dir == 0 ? idx(g, i + 1, j) : idx(g, i, j + 1);
+  const double sp = dir == 0 ? g.dx : g.dy;
+  for (int c = 0; c < 8; ++c) d.c[c][k] += -((f.c[c][kp] - f.c[c][k]) / sp);
+}
+
+__global__ __launch_bounds__(256) void curl_kernel(GridD g, const double* __restrict__ ez, double* __restrict__ dbx,
+                                                   double* __restrict__ dby) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  const std::size_t k = idx(g, i, j);
+  if (j < g.ny) dbx[k] = (ez[k] - ez[idx(g, i, j + 1)]) / g.dy;
+  if (i < g.nx) dby[k] = (ez[idx(g, i + 1, j)] - ez[k]) / g.dx;
+}
+
+// Reset of the step status block by a kernel (no memset node in a captured graph).
+__global__ void status_reset_kernel(StepStatus* s) {
+  const unsigned long long one = static_cast<unsigned long long>(__double_as_longlong(1.0));
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; ++q) s->theta[q] = one;
+    s->live_theta0 = one;
+    s->live_theta1 = one;
+    s->divb_residual = 0;
+    s->divb_scale = 0;
+    s->max_rate = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+
+void launch_eq_flags(const GridD& g, int dir, const IfaceView<double>& in, int* eq, hipStream_t stream) {
+  const dim3 b(64, 4);
+  const int lx = dir == 0 ? g.nx + 1 : g.nx + 2 * g.ng;
+  const int ly = dir == 0 ? g.ny + 2 * g.ng : g.ny + 1;
+  hipLaunchKernelGGL(eq_flag_kernel, dim3((lx + 63) / 64, (ly + 3) / 4), b, 0, stream, g, dir, in, eq);
+}
+
+void launch_flux_difference(const GridD& g, int dir, const FluxView<double>& f, const StateOut& d,
+                            hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(flux_diff_kernel, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), b, 0, stream, g, dir, f, d);
+}
+
+void launch_curl(const GridD& g, const double* ez, double* dbx, double* dby, hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(curl_kernel, dim3((g.nx + 64) / 64, (g.ny + 4) / 4), b, 0, stream, g, ez, dbx, dby);
+}
+
+void launch_status_reset(StepStatus* s, hipStream_t stream) {
+  hipLaunchKernelGGL(status_reset_kernel, dim3(1), dim3(64), 0, stream, s);
+}
+
+namespace {
+inline dim3 blocks2d(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+}  // namespace
+
+void launch_prep(const GridD& g, const StateView& u, double* bxc, double* byc, const StateView& base,
+                 const double* base_bxc, const double* base_byc, int theta_slot, int theta_co, int theta_both,
+                 int live_checks, int solver_owned, StepStatus* status, hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(prep_kernel, blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, b), b, 0, stream, g, u, bxc, byc, base,
+                     base_bxc, base_byc, theta_slot, theta_co, theta_both, live_checks, solver_owned, status);
+}
+
+void launch_collocate(const GridD& g, int axis, int co, const double* face, double* cell, hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(collocate_kernel, blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, b), b, 0, stream, g, axis, co, face,
+                     cell);
+}
+
+void launch_reconstruct(const GridD& g, int dir, int order, double gamma, const Flags4& flags, const StateView& u,
+                        const double* bxc, const double* byc, const IfaceView<double>& out, int* eq,
+                        hipStream_t stream) {
+  const dim3 b(64, 4);
+  const int lx = dir == 0 ? g.nx + 1 : g.nx + 2 * g.ng;
+  const int ly = dir == 0 ? g.ny + 2 * g.ng : g.ny + 1;
+  const int slo = dir == 0 ? flags.side[0] : flags.side[2];
+  const int shi = dir == 0 ? flags.side[1] : flags.side[3];
+  const dim3 grid = blocks2d(lx, ly, b);
+  switch (order) {
+    case 1:
+      hipLaunchKernelGGL((reconstruct_kernel<double, 1>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq);
+      break;
+    case 2:
+      hipLaunchKernelGGL((reconstruct_kernel<double, 2>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq);
+      break;
+    case 5:
+      hipLaunchKernelGGL((reconstruct_kernel<double, 5>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq);
+      break;
+    default:
+      hipLaunchKernelGGL((reconstruct_kernel<double, 7>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq);
+      break;
+  }
+}
+
+void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4& flags, double gamma,
+                 const IfaceView<double>& in, const FluxView<double>& flux, double* rate, hipStream_t stream) {
+  const dim3 b(64, 4);
+  const dim3 grid = blocks2d(g.nx + 1, g.ny + 1, b);
+  const int periodic = dir == 0 ? (flags.side[0] == 0 && flags.side[1] == 0) : (flags.side[2] == 0 && flags.side[3] == 0);
+  switch (order) {
+    case 1:
+      hipLaunchKernelGGL((flux_kernel<double, 1>), grid, b, 0, stream, g, dir, hll_only, periodic, gamma, in, flux, rate);
+      break;
+    case 2:
+      hipLaunchKernelGGL((flux_kernel<double, 2>), grid, b, 0, stream, g, dir, hll_only, periodic, gamma, in, flux, rate);
+      break;
+    case 5:
+      hipLaunchKernelGGL((flux_kernel<double, 5>), grid, b, 0, stream, g, dir, hll_only, periodic, gamma, in, flux, rate);
+      break;
+    default:
+      hipLaunchKernelGGL((flux_kernel<double, 7>), grid, b, 0, stream, g, dir, hll_only, periodic, gamma, in, flux, rate);
+      break;
+  }
+}
+
+void launch_cfl_reduce(const GridD& g, const double* rate_x, const double* rate_y, unsigned long long* max_rate,
+                       hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(cfl_reduce_kernel, blocks2d(g.nx, g.ny, b), b, 0, stream, g, rate_x, rate_y, max_rate);
+}
+
+void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamma, const StateView& u,
+                     const double* bxc, const double* byc, double* cell_ez, hipStream_t stream) {
+  const int pxh = flags.side[1] != 0 && flags.side[1] != 4;
+  const int pyh = flags.side[3] != 0 && flags.side[3] != 4;
+  if (order == 5 || order == 7) {
+    const dim3 b(kEmfBx, kEmfBy);
+    const dim3 grid = blocks2d(g.nx + 1, g.ny + 1, b);
+    if (order == 5)
+      hipLaunchKernelGGL((cell_emf_kernel<double, 5>), grid, b, 0, stream, g, pxh, pyh, gamma, u, bxc, byc, cell_ez);
+    else
+      hipLaunchKernelGGL((cell_emf_kernel<double, 7>), grid, b, 0, stream, g, pxh, pyh, gamma, u, bxc, byc, cell_ez);
+  } else {
+    const dim3 b(64, 4);
+    hipLaunchKernelGGL(cell_emf_lo_kernel, blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, pxh, pyh,
+                       order <= 1 ? 1 : 0, u, bxc, byc, cell_ez);
+  }
+}
+
+void launch_corner(const GridD& g, int order, const Flags4& flags, const FluxView<double>& fx,
+                   const FluxView<double>& fy, const double* cell_ez, const int* eqx, const int* eqy, double* ez,
+                   hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL((corner_kernel<double>), blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, order, flags, fx, fy,
+                     cell_ez, eqx, eqy, ez);
+}
+
+void launch_update(const GridD& g, double a, double b_, double cw, const double* dt_dev, const StateView& un,
+                   const StateView& us, const FluxView<double>& fx, const FluxView<double>& fy, const double* ez,
+                   const StateOut& out, hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL((update_kernel<double>), blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, a, b_, cw, dt_dev, un,
+                     us, fx, fy, ez, out);
+}
+
+void launch_rk_combine(const GridD& g, double a, double b_, double c, const StateView& un, const StateView& us,
+                       const StateView& dudt, const StateOut& out, hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(rk_combine_kernel, blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, a, b_, c, un, us, dudt, out);
+}
+
+void launch_ghosts(const GridD& g, const Flags4& fluid, const Flags4& field, const StateOut& u, hipStream_t stream) {
+  for (int axis = 0; axis < 2; ++axis) {
+    const int lines = axis == 0 ? g.ny + 2 * g.ng : g.nx + 2 * g.ng;
+    const int total = 2 * g.ng * lines;
+    hipLaunchKernelGGL(ghost_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, g, axis, fluid, field, u);
+  }
+}
+
+void launch_dt(const unsigned long long* max_rate_bits, double effective_cfl, double dx, double dy, double* dt_out,
+               double* t_accum, hipStream_t stream) {
+  hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(1), 0, stream, max_rate_bits, effective_cfl, dx, dy, dt_out, t_accum);
+}
+
+void launch_seed(const GridD& g, const IcSpecD& s, const StateOut& out, double* raw_mag, int* status,
+                 hipStream_t stream) {
+  const dim3 b(64, 4);
+  const dim3 grid = blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, b);
+  hipLaunchKernelGGL(seed_kernel, grid, b, 0, stream, g, s, out, raw_mag, status);
+  hipLaunchKernelGGL(recollocate_kernel, grid, b, 0, stream, g, s.gamma, out, raw_mag, status);
+}
+
+void launch_divb_linf(const GridD& g, const StateView& u, unsigned long long* out, hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(divb_linf_kernel, blocks2d(g.nx, g.ny, b), b, 0, stream, g, u, out);
+}
+
+void launch_theta(const GridD& g, const StateView& base, const StateView& cand, int co, unsigned long long* out,
+                  hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(theta_kernel, blocks2d(g.nx, g.ny, b), b, 0, stream, g, base, cand, co, out);
+}
+
+void launch_cell_sums(const GridD& g, const StateView& u, double* partials, int blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(cell_sums_kernel, dim3(blocks), dim3(256), 0, stream, g, u, partials);
+}
+
+// Transverse Gauss rows and the outer-ghost one-sided face->cell rows.  The
+// one-sided row integrates the degree-(w-1) interpolant through w equally
+// spaced face samples over one cell by 4-point Gauss-Legendre quadrature of
+// its barycentric form; evaluated in long double and rounded once.
+void build_quad_rows(QuadRows& r) {
+  const double p5[3][5] = {{-0.0392523473402346598, 0.312987195324173573, 0.910833333333333384,
+                            -0.216320528657506839, 0.0317523473402346601},
+                           {0.00468749999999999983, -0.0604166666666666671, 1.11145833333333344,
+                            -0.0604166666666666671, 0.00468749999999999983},
+                           {0.0317523473402346601, -0.216320528657506839, 0.910833333333333384,
+                            0.312987195324173573, -0.0392523473402346598}};
+  const double p7[4][7] = {{0.00860109764472105458, -0.0796691475531970394, 0.415576982717296983,
+                            0.849155569794037945, -0.245805403605616202, 0.0591434746546156687,
+                            -0.0070025736518584264},
+                           {0.00327670662777140203, -0.0266359260617751309, 0.0928282644451935562,
+                            1.08046056237311472, -0.183384586025290314, 0.0375843394851378479,
+                            -0.00412936084415200903},
+                           {-0.00412936084415200903, 0.0375843394851378479, -0.183384586025290314,
+                            1.08046056237311472, 0.0928282644451935562, -0.0266359260617751309,
+                            0.00327670662777140203},
+                           {-0.0070025736518584264, 0.0591434746546156687, -0.245805403605616202,
+                            0.849155569794037945, 0.415576982717296983, -0.0796691475531970394,
+                            0.00860109764472105458}};
+  for (int q = 0; q < 3; ++q)
+    for (int t = 0; t < 5; ++t) r.mp5_point[q][t] = p5[q][t];
+  for (int q = 0; q < 4; ++q)
+    for (int t = 0; t < 7; ++t) r.mp7_point[q][t] = p7[q][t];
+  r.mp5_gauss[0] = 5.0 / 18.0;
+  r.mp5_gauss[1] = 4.0 / 9.0;
+  r.mp5_gauss[2] = 5.0 / 18.0;
+  r.mp7_gauss[0] = 0.173927422568726925;
+  r.mp7_gauss[1] = 0.326072577431273103;
+  r.mp7_gauss[2] = 0.326072577431273103;
+  r.mp7_gauss[3] = 0.173927422568726925;
+  const long double xq[4] = {0.0694318442029737124L, 0.3300094782075718676L, 0.6699905217924281324L,
+                             0.9305681557970262876L};
+  const long double wq[4] = {0.1739274225687269287L, 0.3260725774312730713L, 0.3260725774312730713L,
+                             0.1739274225687269287L};
+  const int widths[4] = {8, 6, 4, 2};
+  for (int wi = 0; wi < 4; ++wi) {
+    const int w = widths[wi];
+    long double bary[8];
+    for (int k = 0; k < w; ++k) {
+      long double c = 1;  // (-1)^k C(w-1, k)
+      for (int t = 0; t < k; ++t) c = c * (w - 1 - t) / (t + 1);
+      bary[k] = (k % 2 == 0) ? c : -c;
+    }
+    for (int rel = 0; rel < 8; ++rel) {
+      for (int k = 0; k < 8; ++k) r.one_sided[wi][rel][k] = 0.0;
+      if (rel >= w) continue;
+      long double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int q = 0; q < 4; ++q) {
+        const long double x = rel + xq[q];
+        long double den = 0;
+        for (int k = 0; k < w; ++k) den += bary[k] / (x - k);
+        for (int k = 0; k < w; ++k) acc[k] += wq[q] * (bary[k] / (x - k)) / den;
+      }
+      for (int k = 0; k < w; ++k) r.one_sided[wi][rel][k] = static_cast<double>(acc[k]);
+    }
+  }
+}
+
+hipError_t upload_quad_rows(const QuadRows& rows) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_rows), &rows, sizeof(QuadRows));
+}
+
+}  // namespace qmhd

@@ -1,0 +1,304 @@
+"""Parity of the gfx950 kernels with the CPU oracle, through the C ABI.
+
+Tolerances: the kernels keep every branch of the reference algorithm but
+evaluate short dot products with fused multiply-adds where the reference (and
+the oracle) use correctly rounded exact expansions, so each operation differs
+by a few ulp.  Launch-level outputs are compared in relative L1 (<= 1e-13) and
+whole runs per conserved variable in relative L1 (<= 1e-12, the bar for the
+plain FP64 form); integer/copy operations (ghost fills) are compared bit for
+bit."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from gpu_helpers import (call, flags, interior, ptrs, qgrid, rel_l1, rel_l1_state, scratch, to_dev,
+                         to_host, zeros_dev)
+from oracle_ctypes import OracleSolver, ic_spec
+
+pytestmark = pytest.mark.gpu
+
+G53 = 5.0 / 3.0
+
+
+def lib():
+    from quasar_amd import _lib
+    return _lib.load()
+
+
+def ot_planes(oracle, n, ng=4, noise=0.0):
+    g = oracle.grid(n, n, ng)
+    spec = ic_spec("orszag_tang", G53, b_uniform=[1 / np.sqrt(4 * np.pi), 0, 0],
+                   ot_rho=25 / (36 * np.pi), ot_p=5 / (12 * np.pi), noise_amplitude=noise, noise_seed=0)
+    st, planes = oracle.seed(g, spec)
+    assert st == 0
+    oracle.fill_ghosts(g, (0, 0, 0, 0), (0, 0, 0, 0), planes)
+    return g, planes
+
+
+def face_mask(nx, ny, ng, d):
+    m = np.zeros((ny + 2 * ng, nx + 2 * ng), dtype=bool)
+    if d == 0:
+        m[:, ng:ng + nx + 1] = True
+    else:
+        m[ng:ng + ny + 1, :] = True
+    return m.ravel()
+
+
+def flux_mask(nx, ny, ng):
+    m = np.zeros((ny + 2 * ng, nx + 2 * ng), dtype=bool)
+    m[ng:ng + ny + 1, ng:ng + nx + 1] = True
+    return m.ravel()
+
+
+@pytest.mark.parametrize("kind", ["orszag_tang", "brio_wu", "rotor"])
+def test_seed_matches_oracle(oracle, kind):
+    n, ng = 24, 4
+    g = oracle.grid(n, n, ng)
+    kw = dict(orszag_tang=dict(b_uniform=[1 / np.sqrt(4 * np.pi), 0, 0], ot_rho=25 / (36 * np.pi),
+                               ot_p=5 / (12 * np.pi), noise_amplitude=1e-3, noise_seed=7),
+              brio_wu=dict(left=[1, 1, 0, 0, 0, 0.75, 1, 0], right=[0.125, 0.1, 0, 0, 0, 0.75, -1, 0]),
+              rotor=dict(center=[0.5, 0.5], b_uniform=[5 / np.sqrt(4 * np.pi), 0, 0], p_ambient=1.0))[kind]
+    gamma = {"orszag_tang": G53, "brio_wu": 2.0, "rotor": 1.4}[kind]
+    spec = ic_spec(kind, gamma, **kw)
+    st, ref = oracle.seed(g, spec)
+    assert st == 0
+    from quasar_amd import _lib
+    qs = _lib.QIcSpec()
+    C.memmove(C.byref(qs), C.byref(spec), C.sizeof(qs))
+    qg = qgrid(n, n, ng)
+    out = zeros_dev(len(ref[0]), 8)
+    call(lib().qmhd_seed_initial_state, C.byref(qg), C.byref(qs), ptrs(out), C.c_void_p(scratch(qg).data_ptr()), None)
+    got = to_host(out)
+    for c in range(8):
+        scale = max(1.0, np.abs(ref[c]).max())
+        assert np.max(np.abs(got[c] - ref[c])) <= 1e-14 * scale, (kind, c)
+
+
+@pytest.mark.parametrize("modes", [(0, 0, 0, 0), (1, 1, 1, 1), (2, 2, 2, 2), (1, 1, 0, 0), (2, 2, 0, 0)])
+def test_ghost_fill_bitwise(oracle, modes):
+    nx, ny, ng = 13, 9, 4
+    rng = np.random.default_rng(1)
+    u = [rng.normal(size=(nx + 2 * ng) * (ny + 2 * ng)) for _ in range(8)]
+    ref = [a.copy() for a in u]
+    oracle.fill_ghosts(oracle.grid(nx, ny, ng), modes, modes, ref)
+    dev = to_dev(u)
+    call(lib().qmhd_fill_ghosts, C.byref(qgrid(nx, ny, ng)), flags(modes), flags(modes), ptrs(dev), None)
+    got = to_host(dev)
+    for c in range(8):
+        np.testing.assert_array_equal(got[c], ref[c])
+
+
+@pytest.mark.parametrize("order,ng", [(7, 4), (5, 3), (2, 2), (1, 4)])
+def test_reconstruct_and_flux_match_oracle(oracle, order, ng):
+    n = 32
+    g, planes = ot_planes(oracle, n, ng, noise=1e-3)
+    qg = qgrid(n, n, ng)
+    npad = len(planes[0])
+    sc = scratch(qg)
+    for d in (0, 1):
+        ref16 = [np.zeros(npad) for _ in range(16)]
+        oracle.reconstruct(g, d, order, (0, 0, 0, 0), G53, planes, ref16)
+        u_dev = to_dev(planes)
+        out = zeros_dev(npad, 16)
+        call(lib().qmhd_reconstruct, C.byref(qg), d, order, flags((0, 0, 0, 0)), G53, ptrs(u_dev), ptrs(out),
+             C.c_void_p(sc.data_ptr()), None)
+        got16 = to_host(out)
+        m = face_mask(n, n, ng, d)
+        gm = [x[m] for x in got16]
+        rm = [x[m] for x in ref16]
+        for side in (0, 8):
+            for c in range(8):
+                assert rel_l1_state(gm, rm, c, side) <= 1e-13, ("reconstruct", d, side + c)
+        # Same interface states into both face-flux implementations.
+        hll = 1 if order <= 1 else 0
+        ref8 = [np.zeros(npad) for _ in range(8)]
+        oracle.face_flux(g, d, order, hll, (0, 0, 0, 0), G53, ref16, ref8)
+        in_dev = to_dev(ref16)
+        fl = zeros_dev(npad, 8)
+        call(lib().qmhd_hlld_flux, C.byref(qg), d, order, hll, flags((0, 0, 0, 0)), G53, ptrs(in_dev), ptrs(fl),
+             C.c_void_p(sc.data_ptr()), None)
+        got8 = to_host(fl)
+        fm = flux_mask(n, n, ng)
+        gf = [x[fm] for x in got8]
+        rf = [x[fm] for x in ref8]
+        for c in range(8):
+            assert rel_l1_state(gf, rf, c) <= 1e-13, ("flux", d, c)
+
+
+@pytest.mark.parametrize("order,ng", [(7, 4), (5, 3), (2, 2)])
+def test_ct_emf_and_cfl_match_oracle(oracle, order, ng):
+    n = 32
+    g, planes = ot_planes(oracle, n, ng, noise=1e-3)
+    npad = len(planes[0])
+    ifx = [np.zeros(npad) for _ in range(16)]
+    ify = [np.zeros(npad) for _ in range(16)]
+    oracle.reconstruct(g, 0, order, (0, 0, 0, 0), G53, planes, ifx)
+    oracle.reconstruct(g, 1, order, (0, 0, 0, 0), G53, planes, ify)
+    fx = [np.zeros(npad) for _ in range(8)]
+    fy = [np.zeros(npad) for _ in range(8)]
+    oracle.face_flux(g, 0, order, 0, (0, 0, 0, 0), G53, ifx, fx)
+    oracle.face_flux(g, 1, order, 0, (0, 0, 0, 0), G53, ify, fy)
+    ref = oracle.ct_emf(g, order, 0, (0, 0, 0, 0), G53, planes, ifx, ify)
+    qg = qgrid(n, n, ng)
+    sc = scratch(qg)
+    ez = zeros_dev(npad, 1)[0]
+    dev = [to_dev(x) for x in (planes, ifx, ify, fx, fy)]
+    call(lib().qmhd_ct_emf, C.byref(qg), order, flags((0, 0, 0, 0)), G53, ptrs(dev[0]), ptrs(dev[1]),
+         ptrs(dev[2]), ptrs(dev[3]), ptrs(dev[4]), C.c_void_p(ez.data_ptr()), C.c_void_p(sc.data_ptr()), None)
+    got = to_host([ez])[0]
+    fm = flux_mask(n, n, ng)
+    assert rel_l1(got[fm], ref["ez"][fm]) <= 1e-13
+    rate = C.c_double()
+    call(lib().qmhd_cfl_max_rate, C.byref(qg), order, flags((0, 0, 0, 0)), G53, ptrs(dev[1]), ptrs(dev[2]),
+         C.byref(rate), C.c_void_p(sc.data_ptr()), None)
+    ref_rate = oracle.cfl_max_rate(g, order, (0, 0, 0, 0), G53, ifx, ify)
+    assert abs(rate.value - ref_rate) <= 1e-14 * ref_rate
+
+
+def test_reductions_match_oracle(oracle):
+    n, ng = 32, 4
+    g, planes = ot_planes(oracle, n, ng, noise=1e-3)
+    qg = qgrid(n, n, ng)
+    sc = scratch(qg)
+    dev = to_dev(planes)
+    cand = [p.copy() for p in planes]
+    cand[4] = cand[4] - 0.3 * np.abs(cand[4])  # drive some cells towards negative pressure
+    cdev = to_dev(cand)
+    for co in (0, 1):
+        th = C.c_double()
+        call(lib().qmhd_admissible_fraction, C.byref(qg), ptrs(dev), ptrs(cdev), co, C.byref(th),
+             C.c_void_p(sc.data_ptr()), None)
+        ref = oracle.admissible_fraction(g, planes, cand, co)
+        assert abs(th.value - ref) <= 1e-12, (co, th.value, ref)
+    v = C.c_double()
+    call(lib().qmhd_divb_linf, C.byref(qg), ptrs(dev), C.byref(v), C.c_void_p(sc.data_ptr()), None)
+    assert abs(v.value - oracle.divb_linf(g, planes)) <= 1e-12
+    m, e = C.c_double(), C.c_double()
+    call(lib().qmhd_conserved_cell_sums, C.byref(qg), ptrs(dev), C.byref(m), C.byref(e),
+         C.c_void_p(sc.data_ptr()), None)
+    rm, re = oracle.conserved_totals(g, planes)
+    assert m.value == rm and e.value == re
+
+
+def solver_pair(oracle, n_x, n_y, ng, gamma, order, planes, fluid=(0, 0, 0, 0), lx=1.0, ly=1.0,
+                recon="mp7"):
+    from quasar_amd import mhd
+    names = {0: "periodic", 1: "outflow", 2: "wall"}
+    cfg = mhd.MhdConfig()
+    cfg.grid = mhd.Grid2D(n_x, n_y, lx, ly, 0.0, 0.0, ng)
+    cfg.gamma = gamma
+    cfg.reconstruction = recon
+    for s in range(4):
+        cfg.boundary.set_fluid_side(s, names[fluid[s]])
+        cfg.boundary.set_field_side(s, names[fluid[s]])
+    gpu = mhd.MhdSolver2D(cfg)
+    for k, name in enumerate(mhd.STATE_COMPONENTS):
+        gpu.seed_state(name, planes[k])
+    ref = OracleSolver(oracle, oracle.grid(n_x, n_y, ng, lx, ly), gamma, 0.4, order, fluid=fluid)
+    ref.seed(planes)
+    return gpu, ref
+
+
+def compare_states(gpu, ref, nx, ny, ng, tol):
+    out = ref.state()
+    names = ("rho", "mx", "my", "mz", "energy", "bx_face", "by_face", "bz_cell")
+    got = [interior(gpu.state_component_to_host(name), nx, ny, ng) for name in names]
+    exp = [interior(out[k], nx, ny, ng) for k in range(8)]
+    errs = {name: rel_l1_state(got, exp, k) for k, name in enumerate(names)}
+    assert all(v <= tol for v in errs.values()), errs
+    return errs
+
+
+def test_orszag_tang_mp7_steps_match_oracle(oracle):
+    n, ng = 48, 4
+    _, planes = ot_planes(oracle, n, ng, noise=1e-3)
+    gpu, ref = solver_pair(oracle, n, n, ng, G53, 7, planes)
+    for _ in range(6):
+        dt_g, dt_r = gpu.cfl_limit(), ref.cfl_limit()
+        assert abs(dt_g - dt_r) <= 1e-13 * dt_r
+        gpu.step_unchecked(dt_r)
+        ref.step_unchecked(dt_r)
+    compare_states(gpu, ref, n, n, ng, 1e-12)
+    assert abs(gpu.divergence_b_max() - ref.divergence_b_max()) <= 1e-12
+
+
+def test_brio_wu_mp5_steps_match_oracle(oracle):
+    nx, ny, ng = 160, 4, 3
+    g = oracle.grid(nx, ny, ng, 1.0, ny / nx)
+    spec = ic_spec("brio_wu", 2.0, left=[1, 1, 0, 0, 0, 0.75, 1, 0], right=[0.125, 0.1, 0, 0, 0, 0.75, -1, 0])
+    st, planes = oracle.seed(g, spec)
+    assert st == 0
+    gpu, ref = solver_pair(oracle, nx, ny, ng, 2.0, 5, planes, fluid=(1, 1, 0, 0), ly=ny / nx, recon="mp5")
+    for _ in range(25):
+        dt_r = ref.cfl_limit()
+        assert abs(gpu.cfl_limit() - dt_r) <= 1e-13 * dt_r
+        gpu.step_unchecked(dt_r)
+        ref.step_unchecked(dt_r)
+    compare_states(gpu, ref, nx, ny, ng, 1e-12)
+
+
+def test_rotor_mp7_outflow_steps_match_oracle(oracle):
+    n, ng = 40, 4
+    g = oracle.grid(n, n, ng)
+    spec = ic_spec("rotor", 1.4, center=[0.5, 0.5], b_uniform=[5 / np.sqrt(4 * np.pi), 0, 0], p_ambient=1.0)
+    st, planes = oracle.seed(g, spec)
+    assert st == 0
+    gpu, ref = solver_pair(oracle, n, n, ng, 1.4, 7, planes, fluid=(1, 1, 1, 1))
+    for _ in range(4):
+        dt_r = ref.cfl_limit()
+        assert abs(gpu.cfl_limit() - dt_r) <= 1e-13 * dt_r
+        gpu.step_unchecked(dt_r)
+        ref.step_unchecked(dt_r)
+    compare_states(gpu, ref, n, n, ng, 1e-12)
+
+
+def test_muscl_wall_steps_match_oracle(oracle):
+    n, ng = 32, 2
+    g = oracle.grid(n, n, ng)
+    spec = ic_spec("blast", 5 / 3, rho_ambient=1.0, p_ambient=0.1, p_core=10.0, r_in=0.1, center=[0.5, 0.5],
+                   b_uniform=[0.5, 0.3, 0.0])
+    st, planes = oracle.seed(g, spec)
+    assert st == 0
+    gpu, ref = solver_pair(oracle, n, n, ng, 5 / 3, 2, planes, fluid=(2, 2, 2, 2), recon="muscl_minmod")
+    for _ in range(4):
+        dt_r = ref.cfl_limit()
+        assert abs(gpu.cfl_limit() - dt_r) <= 1e-13 * dt_r
+        gpu.step_unchecked(dt_r)
+        ref.step_unchecked(dt_r)
+    compare_states(gpu, ref, n, n, ng, 1e-12)
+
+
+def test_invalid_live_states_raise(oracle):
+    n, ng = 16, 4
+    _, planes = ot_planes(oracle, n, ng)
+    bad = [p.copy() for p in planes]
+    bad[4][(3 + ng) + (n + 2 * ng) * (3 + ng)] = 0.0
+    gpu, _ = solver_pair(oracle, n, n, ng, G53, 7, bad)
+    with pytest.raises(ValueError):
+        gpu.cfl_limit()
+    nonsol = [p.copy() for p in planes]
+    nonsol[5][(3 + ng) + (n + 2 * ng) * (3 + ng)] += 0.1
+    gpu2, _ = solver_pair(oracle, n, n, ng, G53, 7, nonsol)
+    with pytest.raises(ValueError):
+        gpu2.cfl_limit()
+    with pytest.raises(ValueError):
+        gpu2.step_unchecked(-1.0)
+
+
+def test_device_run_loop_matches_host_loop(oracle):
+    """The on-device auto-dt loop takes the same steps as cfl_limit + step_unchecked."""
+    n, ng = 32, 4
+    _, planes = ot_planes(oracle, n, ng, noise=1e-3)
+    a, _ = solver_pair(oracle, n, n, ng, G53, 7, planes)
+    b, _ = solver_pair(oracle, n, n, ng, G53, 7, planes)
+    done, t = a.run(5)
+    assert done == 5
+    tb = 0.0
+    for _ in range(5):
+        dt = b.cfl_limit()
+        b.step_unchecked(dt)
+        tb += dt
+    assert abs(t - tb) <= 1e-15 * tb
+    for name in ("rho", "energy", "bx_face"):
+        np.testing.assert_array_equal(a.state_component_to_host(name), b.state_component_to_host(name))
