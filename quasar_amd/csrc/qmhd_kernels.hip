@@ -12,6 +12,7 @@
 // fluxes are of the kernel's arithmetic type T.
 
 #include "qmhd_kernels.hpp"
+#include "qmhd_stencil.hpp"
 
 #include <cstdint>
 
@@ -21,21 +22,6 @@ __constant__ QuadRows c_rows;
 
 // ---------------------------------------------------------------------------
 // Small helpers.
-
-__device__ __forceinline__ std::size_t idx(const GridD& g, int i, int j) {
-  return static_cast<std::size_t>(i + g.ng) + static_cast<std::size_t>(g.pitch) * static_cast<std::size_t>(j + g.ng);
-}
-
-// Periodic wrap or clamp of a face/cell coordinate (mode 4 keeps exchanged guards).
-__device__ __forceinline__ int face_index(int q, int n, int lo_mode, int hi_mode) {
-  if (lo_mode == 0 && hi_mode == 0) {
-    int w = q % n;
-    return w < 0 ? w + n : w;
-  }
-  if (q < 0) return lo_mode == 4 ? q : 0;
-  if (q > n) return hi_mode == 4 ? q : n;
-  return q;
-}
 
 // Non-negative doubles order like their bit patterns: min/max through
 // 64-bit integer atomics is exact and order independent.
@@ -58,76 +44,6 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 __device__ __forceinline__ bool lane_leader() { return (threadIdx.x + blockDim.x * threadIdx.y) % 64 == 0; }
-
-// ---------------------------------------------------------------------------
-// Face -> cell collocation of the staggered normal field.
-
-__device__ __forceinline__ int colloc_width(int extent, int ng) {
-  const int req = ng >= 4 ? 8 : ng >= 3 ? 6 : ng >= 2 ? 4 : 2;
-  if (req >= 8 && extent >= 8) return 8;
-  if (req >= 6 && extent >= 6) return 6;
-  if (req >= 4 && extent >= 4) return 4;
-  return extent >= 2 ? 2 : 1;
-}
-
-// Sample of the face array along `axis` at coordinate q on line `fixed`.
-__device__ __forceinline__ double axis_face(const GridD& g, const double* f, int axis, int q, int fixed) {
-  return axis == 0 ? f[idx(g, q, fixed)] : f[idx(g, fixed, q)];
-}
-
-// High-order cell average (8/6/4/2 taps, one-sided rows in the outer ghosts).
-__device__ double face_to_cell(const GridD& g, const double* f, int axis, int cell, int fixed) {
-  const int n = axis == 0 ? g.nx : g.ny;
-  const int width = colloc_width(n + 2 * g.ng, g.ng);
-  const int lo = -g.ng, hi = n + g.ng - 1;
-  if (width == 1) return axis_face(g, f, axis, cell, fixed);
-  const int start_c = cell - (width / 2 - 1);
-  if (start_c >= lo && start_c + width - 1 <= hi) {
-    if (width == 8) {
-      const double s = -191.0 * axis_face(g, f, axis, start_c, fixed) +
-                       1879.0 * axis_face(g, f, axis, start_c + 1, fixed) -
-                       9531.0 * axis_face(g, f, axis, start_c + 2, fixed) +
-                       68323.0 * axis_face(g, f, axis, start_c + 3, fixed) +
-                       68323.0 * axis_face(g, f, axis, start_c + 4, fixed) -
-                       9531.0 * axis_face(g, f, axis, start_c + 5, fixed) +
-                       1879.0 * axis_face(g, f, axis, start_c + 6, fixed) -
-                       191.0 * axis_face(g, f, axis, start_c + 7, fixed);
-      return s / 120960.0;
-    }
-    if (width == 6) {
-      const double s = 11.0 * axis_face(g, f, axis, start_c, fixed) -
-                       93.0 * axis_face(g, f, axis, start_c + 1, fixed) +
-                       802.0 * axis_face(g, f, axis, start_c + 2, fixed) +
-                       802.0 * axis_face(g, f, axis, start_c + 3, fixed) -
-                       93.0 * axis_face(g, f, axis, start_c + 4, fixed) +
-                       11.0 * axis_face(g, f, axis, start_c + 5, fixed);
-      return s / 1440.0;
-    }
-    if (width == 4) {
-      const double s = -axis_face(g, f, axis, start_c, fixed) + 13.0 * axis_face(g, f, axis, start_c + 1, fixed) +
-                       13.0 * axis_face(g, f, axis, start_c + 2, fixed) - axis_face(g, f, axis, start_c + 3, fixed);
-      return s / 24.0;
-    }
-    const double a = axis_face(g, f, axis, start_c, fixed), b = axis_face(g, f, axis, start_c + 1, fixed);
-    return a == b ? a : (a + b) / 2.0;
-  }
-  int start = start_c < lo ? lo : start_c;
-  if (start + width - 1 > hi) start = hi - width + 1;
-  const int wi = width == 8 ? 0 : width == 6 ? 1 : width == 4 ? 2 : 3;
-  const double* row = c_rows.one_sided[wi][cell - start];
-  double s = 0.0;
-  for (int k = 0; k < width; ++k) s += row[k] * axis_face(g, f, axis, start + k, fixed);
-  return s;
-}
-
-// First-order (two bounding faces) cell average used by the low-order retry.
-__device__ __forceinline__ double face_to_cell_lo(const GridD& g, const double* f, int axis, int cell, int fixed) {
-  const int n = axis == 0 ? g.nx : g.ny;
-  const double a = axis_face(g, f, axis, cell, fixed);
-  if (cell >= n + g.ng - 1) return a;
-  const double b = axis_face(g, f, axis, cell + 1, fixed);
-  return a == b ? a : (a + b) / 2.0;
-}
 
 // Internal energy q = E - K - M of a convex combination (the admissible set
 // rho > 0, q > 0 is convex).
@@ -284,166 +200,6 @@ __global__ __launch_bounds__(256) void collocate_kernel(GridD g, int axis, int c
   cell[idx(g, i, j)] = v;
 }
 
-// ---------------------------------------------------------------------------
-// Reconstruction.
-
-template <class T>
-__device__ __forceinline__ St<T> load_cell(const GridD& g, const StateView& u, const double* bxc,
-                                           const double* byc, int i, int j) {
-  const std::size_t k = idx(g, i, j);
-  return St<T>{T(u.c[0][k]), T(u.c[1][k]), T(u.c[2][k]), T(u.c[3][k]),
-               T(u.c[4][k]), T(bxc[k]),    T(byc[k]),    T(u.c[7][k])};
-}
-
-template <class T>
-__device__ __forceinline__ St<T> load_cell_lo(const GridD& g, const StateView& u, int i, int j) {
-  const std::size_t k = idx(g, i, j);
-  return St<T>{T(u.c[0][k]), T(u.c[1][k]), T(u.c[2][k]), T(u.c[3][k]), T(u.c[4][k]),
-               T(face_to_cell_lo(g, u.c[5], 0, i, j)), T(face_to_cell_lo(g, u.c[6], 1, j, i)), T(u.c[7][k])};
-}
-
-template <class T>
-__device__ __forceinline__ bool admissible_state(const St<T>& s, T gm1) {
-  if (!(fin(s.rho) && s.rho > T(0))) return false;
-  const T inv = T(1) / s.rho;
-  const T vx = s.mx * inv, vy = s.my * inv, vz = s.mz * inv;
-  const T p = pressure(s, gm1);
-  return fin(vx) && fin(vy) && fin(vz) && fin(p) && p > T(0) && fin(s.mx) && fin(s.my) && fin(s.mz) &&
-         fin(s.en) && fin(s.bx) && fin(s.by) && fin(s.bz);
-}
-
-template <class T>
-__device__ __forceinline__ void set_normal_b(St<T>& s, int dir, T v) {
-  T& bn = dir == 0 ? s.bx : s.by;
-  s.en = fma(-T(0.5) * bn, bn, fma(T(0.5) * v, v, s.en));
-  bn = v;
-}
-
-template <class T>
-__device__ __forceinline__ T limited_slope(T back, T fwd, int drop) {
-  if (drop < 0) return fwd;
-  if (drop > 0) return back;
-  return minmod2(back, fwd);
-}
-
-__device__ __forceinline__ int slope_drop(int c, int n, int lo, int hi) {
-  if (lo != 0 && lo != 4 && c <= 0) return -1;
-  if (hi != 0 && hi != 4 && c >= n - 1) return +1;
-  return 0;
-}
-
-// Second-order MUSCL-minmod in primitive variables (also the MP fallback).
-template <class T>
-__device__ __noinline__ void muscl_face(const GridD& g, int dir, int i, int j, T gamma, int side_lo, int side_hi,
-                                        const StateView& u, const double* bxc, const double* byc, St<T>& ul,
-                                        St<T>& ur) {
-  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
-  const T gm1 = gamma - T(1);
-  St<T> c[4];
-  for (int s = 0; s < 4; ++s) c[s] = load_cell<T>(g, u, bxc, byc, i + (s - 2) * di, j + (s - 2) * dj);
-  T w[4][8];
-  for (int s = 0; s < 4; ++s) {
-    const T inv = T(1) / c[s].rho;
-    w[s][0] = c[s].rho;
-    w[s][1] = c[s].mx * inv;
-    w[s][2] = c[s].my * inv;
-    w[s][3] = c[s].mz * inv;
-    w[s][4] = pressure(c[s], gm1);
-    w[s][5] = c[s].bx;
-    w[s][6] = c[s].by;
-    w[s][7] = c[s].bz;
-  }
-  const int n_dir = dir == 0 ? g.nx : g.ny;
-  const int drop_m = slope_drop((dir == 0 ? i : j) - 1, n_dir, side_lo, side_hi);
-  const int drop_p = slope_drop(dir == 0 ? i : j, n_dir, side_lo, side_hi);
-  T wl[8], wr[8];
-  for (int q = 0; q < 8; ++q) {
-    wl[q] = w[1][q] + T(0.5) * limited_slope(w[1][q] - w[0][q], w[2][q] - w[1][q], drop_m);
-    wr[q] = w[2][q] - T(0.5) * limited_slope(w[2][q] - w[1][q], w[3][q] - w[2][q], drop_p);
-  }
-  auto to_cons = [&](const T* p) {
-    St<T> s;
-    s.rho = p[0];
-    s.mx = p[0] * p[1];
-    s.my = p[0] * p[2];
-    s.mz = p[0] * p[3];
-    s.bx = p[5];
-    s.by = p[6];
-    s.bz = p[7];
-    s.en = p[4] / gm1 + p[0] * half_sq(p[1], p[2], p[3]) + half_sq(p[5], p[6], p[7]);
-    return s;
-  };
-  ul = to_cons(wl);
-  ur = to_cons(wr);
-}
-
-template <class T, int ORDER>
-__device__ __forceinline__ void mp_char_face(const GridD& g, int dir, int i, int j, T gamma, const StateView& u,
-                                             const double* bxc, const double* byc, St<T>& ul, St<T>& ur) {
-  constexpr int H = ORDER == 5 ? 2 : 3;
-  constexpr int UW = 2 * H + 2;
-  constexpr int W = 2 * H + 1;
-  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
-  St<T> cells[UW];
-#pragma unroll
-  for (int q = 0; q < UW; ++q) cells[q] = rot(load_cell<T>(g, u, bxc, byc, i + (q - 1 - H) * di, j + (q - 1 - H) * dj), dir);
-  St<T> ref;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) set_comp(ref, c, T(0.5) * comp(cells[H], c) + T(0.5) * comp(cells[H + 1], c));
-  Eig<T> e;
-  eig_build(ref, gamma, e);
-  // Deltas in the 7-vector (rho, mn, mt1, mt2, E, bt1, bt2).
-  T d[UW][7];
-#pragma unroll
-  for (int q = 0; q < UW; ++q) {
-    d[q][0] = cells[q].rho - ref.rho;
-    d[q][1] = cells[q].mx - ref.mx;
-    d[q][2] = cells[q].my - ref.my;
-    d[q][3] = cells[q].mz - ref.mz;
-    d[q][4] = cells[q].en - ref.en;
-    d[q][5] = cells[q].by - ref.by;
-    d[q][6] = cells[q].bz - ref.bz;
-  }
-  T aL[7], aR[7];
-#pragma unroll
-  for (int v = 0; v < 7; ++v) {
-    aL[v] = T(0);
-    aR[v] = T(0);
-  }
-#pragma unroll
-  for (int k = 0; k < 7; ++k) {
-    T w[UW];
-#pragma unroll
-    for (int q = 0; q < UW; ++q) {
-      T acc = T(0);
-#pragma unroll
-      for (int v = 0; v < 7; ++v) acc += e.L[k][v] * d[q][v];
-      w[q] = acc;
-    }
-    // Left state: cells 0..W-1 (centre H), right state: reversed cells 1..W.
-    T sl[W], sr[W];
-#pragma unroll
-    for (int s = 0; s < W; ++s) {
-      sl[s] = w[s];
-      sr[s] = w[W - s];
-    }
-    const T fl = ORDER == 5 ? mp5_face(sl) : mp7_face(sl);
-    const T fr = ORDER == 5 ? mp5_face(sr) : mp7_face(sr);
-    constexpr int o = ORDER == 5 ? 0 : 1;
-    const T wl = mp_limit(fl, sl[o], sl[o + 1], sl[o + 2], sl[o + 3], sl[o + 4]);
-    const T wr = mp_limit(fr, sr[o], sr[o + 1], sr[o + 2], sr[o + 3], sr[o + 4]);
-#pragma unroll
-    for (int v = 0; v < 7; ++v) {
-      aL[v] += e.R[v][k] * wl;
-      aR[v] += e.R[v][k] * wr;
-    }
-  }
-  St<T> sL{ref.rho + aL[0], ref.mx + aL[1], ref.my + aL[2], ref.mz + aL[3], ref.en + aL[4], ref.bx + T(0), ref.by + aL[5], ref.bz + aL[6]};
-  St<T> sR{ref.rho + aR[0], ref.mx + aR[1], ref.my + aR[2], ref.mz + aR[3], ref.en + aR[4], ref.bx + T(0), ref.by + aR[5], ref.bz + aR[6]};
-  ul = rot(sL, dir);
-  ur = rot(sR, dir);
-}
-
 // One thread per face normal to `dir`: dir 0 faces i in [0,nx], j in [-ng,ny+ng);
 // dir 1 faces i in [-ng,nx+ng), j in [0,ny].  Writes L (0..7) and R (8..15)
 // interface planes and the per-face "L == R" flag used by the CT 1-D shortcut.
@@ -461,38 +217,8 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(GridD g, int dir, T ga
   } else {
     if (li >= g.nx + 2 * g.ng || lj > g.ny) return;
   }
-  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
-  const T gm1 = gamma - T(1);
   St<T> ul, ur;
-  if (ORDER <= 1) {
-    ul = load_cell_lo<T>(g, u, i - di, j - dj);
-    ur = load_cell_lo<T>(g, u, i, j);
-  } else if (ORDER == 2) {
-    muscl_face<T>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
-  } else {
-    mp_char_face<T, ORDER>(g, dir, i, j, gamma, u, bxc, byc, ul, ur);
-    if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1))
-      muscl_face<T>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
-  }
-  const T bn = T((dir == 0 ? u.c[5] : u.c[6])[idx(g, i, j)]);
-  set_normal_b(ul, dir, bn);
-  set_normal_b(ur, dir, bn);
-  if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1)) {
-    if (ORDER <= 1) {
-      ul = load_cell_lo<T>(g, u, i - di, j - dj);
-      ur = load_cell_lo<T>(g, u, i, j);
-    } else {
-      ul = load_cell<T>(g, u, bxc, byc, i - di, j - dj);
-      ur = load_cell<T>(g, u, bxc, byc, i, j);
-    }
-    set_normal_b(ul, dir, bn);
-    set_normal_b(ur, dir, bn);
-  }
-  if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1)) {
-    const T nan = Lim<T>::nan();
-    ul = St<T>{nan, nan, nan, nan, nan, nan, nan, nan};
-    ur = ul;
-  }
+  reconstruct_face<T, ORDER>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
   const std::size_t k = idx(g, i, j);
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
@@ -663,12 +389,6 @@ __global__ __launch_bounds__(256) void cfl_reduce_kernel(GridD g, const double* 
 // from a 5x5 stencil, MP7: 4x4 from 7x7).  The block stages its cell stencil
 // (8 components) in LDS.
 
-template <class T>
-__device__ __forceinline__ T physical_ez(T rho, T mx, T my, T bx, T by) {
-  const T inv = T(1) / rho;
-  return (my * inv) * bx - (mx * inv) * by;
-}
-
 constexpr int kEmfBx = 16;
 constexpr int kEmfBy = 16;
 
@@ -812,25 +532,6 @@ __global__ __launch_bounds__(256) void cell_emf_lo_kernel(GridD g, int phys_x_hi
 // the cell-average -> corner tensor interpolation of the cell E_z, plus the
 // exact one-dimensional shortcuts.
 
-__device__ __forceinline__ double edge_interp(int order, const double* e) {
-  if (order >= 7) {
-    bool constant = true;
-#pragma unroll
-    for (int k = 1; k < 8; ++k) constant = constant && e[k] == e[0];
-    if (constant) return e[0];
-    return (-3.0 * e[0] + 29.0 * e[1] - 139.0 * e[2] + 533.0 * e[3] + 533.0 * e[4] - 139.0 * e[5] + 29.0 * e[6] -
-            3.0 * e[7]) / 840.0;
-  }
-  if (order >= 5) {
-    bool constant = true;
-#pragma unroll
-    for (int k = 1; k < 6; ++k) constant = constant && e[k] == e[0];
-    if (constant) return e[0];
-    return (e[0] - 8.0 * e[1] + 37.0 * e[2] + 37.0 * e[3] - 8.0 * e[4] + e[5]) / 60.0;
-  }
-  return e[0] == e[1] ? e[0] : (e[0] + e[1]) / 2.0;
-}
-
 template <class T>
 __global__ __launch_bounds__(256) void corner_kernel(GridD g, int order, Flags4 m, FluxView<T> fx, FluxView<T> fy,
                                                      const double* __restrict__ cell_ez, const int* __restrict__ eqx,
@@ -838,68 +539,7 @@ __global__ __launch_bounds__(256) void corner_kernel(GridD g, int order, Flags4 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.nx || j > g.ny) return;
-  const std::size_t k = idx(g, i, j);
-  const bool conducting = (i == 0 && (m.side[0] == 2 || m.side[0] == 3)) || (i == g.nx && m.side[1] == 2) ||
-                          (j == 0 && m.side[2] == 2) || (j == g.ny && m.side[3] == 2);
-  if (conducting) {
-    ez[k] = 0.0;
-    return;
-  }
-  const int lo = order >= 7 ? -4 : order >= 5 ? -3 : -1;
-  const int nt = order >= 7 ? 8 : order >= 5 ? 6 : 2;
-  const int rh = order >= 7 ? 3 : order >= 5 ? 2 : 0;  // transverse recovery half-width
-  auto mi = [&](int q) { return face_index(q, g.nx, m.side[0], m.side[1]); };
-  auto mj = [&](int q) { return face_index(q, g.ny, m.side[2], m.side[3]); };
-  double e[8];
-  for (int t = 0; t < nt; ++t) e[t] = -static_cast<double>(fx.c[6][idx(g, mi(i), mj(j + lo + t))]);
-  const double ex = edge_interp(order, e);
-  for (int t = 0; t < nt; ++t) e[t] = static_cast<double>(fy.c[5][idx(g, mi(i + lo + t), mj(j))]);
-  const double ey = edge_interp(order, e);
-  double line[8];
-  for (int q = 0; q < nt; ++q) {
-    for (int t = 0; t < nt; ++t) e[t] = cell_ez[idx(g, mi(i + lo + t), mj(j + lo + q))];
-    line[q] = edge_interp(order, e);
-  }
-  const double centered = edge_interp(order, line);
-  // x-only data: every y-face problem in the stencil has no jump and the cell
-  // E_z is constant along y; y-only symmetrically.
-  // Face `t` of the interpolation stencil is mapped into the face table first;
-  // its point-recovery problem then spans offsets -rh..rh about that mapped
-  // face, mapped again (periodic wrap or physical clamp).
-  auto no_jump_line = [&](const int* eq, int d) {
-    for (int t = 0; t < nt; ++t) {
-      const int fi = d == 0 ? mi(i) : mi(i + lo + t);
-      const int fj = d == 0 ? mj(j + lo + t) : mj(j);
-      for (int off = -rh; off <= rh; ++off) {
-        const int ii = d == 0 ? mi(fi) : mi(fi + off);
-        const int jj = d == 0 ? mj(fj + off) : mj(fj);
-        if (ii < -g.ng || ii >= g.nx + g.ng || jj < -g.ng || jj >= g.ny + g.ng || eq[idx(g, ii, jj)] == 0)
-          return false;
-      }
-    }
-    return true;
-  };
-  auto constant_along = [&](int axis) {
-    for (int v = 0; v < nt; ++v) {
-      const double ref = axis == 0 ? cell_ez[idx(g, mi(i + lo), mj(j + lo + v))] : cell_ez[idx(g, mi(i + lo + v), mj(j + lo))];
-      for (int w = 1; w < nt; ++w) {
-        const double c = axis == 0 ? cell_ez[idx(g, mi(i + lo + w), mj(j + lo + v))]
-                                   : cell_ez[idx(g, mi(i + lo + v), mj(j + lo + w))];
-        if (c != ref) return false;
-      }
-    }
-    return true;
-  };
-  const bool x_only = no_jump_line(eqy, 1) && constant_along(1);
-  const bool y_only = no_jump_line(eqx, 0) && constant_along(0);
-  double r;
-  if (x_only && !y_only)
-    r = ex;
-  else if (y_only && !x_only)
-    r = ey;
-  else
-    r = (ex + ey) - centered;
-  ez[k] = r;
+  ez[idx(g, i, j)] = corner_ez<T>(g, order, m, fx, fy, cell_ez, eqx, eqy, i, j);
 }
 
 // ---------------------------------------------------------------------------
@@ -1552,64 +1192,6 @@ void launch_cell_sums(const GridD& g, const StateView& u, double* partials, int 
 // one-sided row integrates the degree-(w-1) interpolant through w equally
 // spaced face samples over one cell by 4-point Gauss-Legendre quadrature of
 // its barycentric form; evaluated in long double and rounded once.
-void build_quad_rows(QuadRows& r) {
-  const double p5[3][5] = {{-0.0392523473402346598, 0.312987195324173573, 0.910833333333333384,
-                            -0.216320528657506839, 0.0317523473402346601},
-                           {0.00468749999999999983, -0.0604166666666666671, 1.11145833333333344,
-                            -0.0604166666666666671, 0.00468749999999999983},
-                           {0.0317523473402346601, -0.216320528657506839, 0.910833333333333384,
-                            0.312987195324173573, -0.0392523473402346598}};
-  const double p7[4][7] = {{0.00860109764472105458, -0.0796691475531970394, 0.415576982717296983,
-                            0.849155569794037945, -0.245805403605616202, 0.0591434746546156687,
-                            -0.0070025736518584264},
-                           {0.00327670662777140203, -0.0266359260617751309, 0.0928282644451935562,
-                            1.08046056237311472, -0.183384586025290314, 0.0375843394851378479,
-                            -0.00412936084415200903},
-                           {-0.00412936084415200903, 0.0375843394851378479, -0.183384586025290314,
-                            1.08046056237311472, 0.0928282644451935562, -0.0266359260617751309,
-                            0.00327670662777140203},
-                           {-0.0070025736518584264, 0.0591434746546156687, -0.245805403605616202,
-                            0.849155569794037945, 0.415576982717296983, -0.0796691475531970394,
-                            0.00860109764472105458}};
-  for (int q = 0; q < 3; ++q)
-    for (int t = 0; t < 5; ++t) r.mp5_point[q][t] = p5[q][t];
-  for (int q = 0; q < 4; ++q)
-    for (int t = 0; t < 7; ++t) r.mp7_point[q][t] = p7[q][t];
-  r.mp5_gauss[0] = 5.0 / 18.0;
-  r.mp5_gauss[1] = 4.0 / 9.0;
-  r.mp5_gauss[2] = 5.0 / 18.0;
-  r.mp7_gauss[0] = 0.173927422568726925;
-  r.mp7_gauss[1] = 0.326072577431273103;
-  r.mp7_gauss[2] = 0.326072577431273103;
-  r.mp7_gauss[3] = 0.173927422568726925;
-  const long double xq[4] = {0.0694318442029737124L, 0.3300094782075718676L, 0.6699905217924281324L,
-                             0.9305681557970262876L};
-  const long double wq[4] = {0.1739274225687269287L, 0.3260725774312730713L, 0.3260725774312730713L,
-                             0.1739274225687269287L};
-  const int widths[4] = {8, 6, 4, 2};
-  for (int wi = 0; wi < 4; ++wi) {
-    const int w = widths[wi];
-    long double bary[8];
-    for (int k = 0; k < w; ++k) {
-      long double c = 1;  // (-1)^k C(w-1, k)
-      for (int t = 0; t < k; ++t) c = c * (w - 1 - t) / (t + 1);
-      bary[k] = (k % 2 == 0) ? c : -c;
-    }
-    for (int rel = 0; rel < 8; ++rel) {
-      for (int k = 0; k < 8; ++k) r.one_sided[wi][rel][k] = 0.0;
-      if (rel >= w) continue;
-      long double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      for (int q = 0; q < 4; ++q) {
-        const long double x = rel + xq[q];
-        long double den = 0;
-        for (int k = 0; k < w; ++k) den += bary[k] / (x - k);
-        for (int k = 0; k < w; ++k) acc[k] += wq[q] * (bary[k] / (x - k)) / den;
-      }
-      for (int k = 0; k < w; ++k) r.one_sided[wi][rel][k] = static_cast<double>(acc[k]);
-    }
-  }
-}
-
 hipError_t upload_quad_rows(const QuadRows& rows) {
   return hipMemcpyToSymbol(HIP_SYMBOL(c_rows), &rows, sizeof(QuadRows));
 }

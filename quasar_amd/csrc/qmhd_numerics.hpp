@@ -22,7 +22,7 @@ struct St {
 };
 
 template <class T>
-__device__ __forceinline__ T comp(const St<T>& s, int c) {
+__host__ __device__ __forceinline__ T comp(const St<T>& s, int c) {
   switch (c) {
     case 0: return s.rho;
     case 1: return s.mx;
@@ -36,7 +36,7 @@ __device__ __forceinline__ T comp(const St<T>& s, int c) {
 }
 
 template <class T>
-__device__ __forceinline__ void set_comp(St<T>& s, int c, T v) {
+__host__ __device__ __forceinline__ void set_comp(St<T>& s, int c, T v) {
   switch (c) {
     case 0: s.rho = v; break;
     case 1: s.mx = v; break;
@@ -55,25 +55,25 @@ template <>
 struct Lim<double> {
   static constexpr double eps = 2.220446049250313080847e-16;
   static constexpr double tiny = 2.2250738585072014e-308;
-  static __device__ __forceinline__ double inf() { return __builtin_huge_val(); }
-  static __device__ __forceinline__ double nan() { return __builtin_nan(""); }
+  static __host__ __device__ __forceinline__ double inf() { return __builtin_huge_val(); }
+  static __host__ __device__ __forceinline__ double nan() { return __builtin_nan(""); }
 };
 template <>
 struct Lim<float> {
   static constexpr float eps = 1.1920928955078125e-7f;
   static constexpr float tiny = 1.17549435e-38f;
-  static __device__ __forceinline__ float inf() { return __builtin_huge_valf(); }
-  static __device__ __forceinline__ float nan() { return __builtin_nanf(""); }
+  static __host__ __device__ __forceinline__ float inf() { return __builtin_huge_valf(); }
+  static __host__ __device__ __forceinline__ float nan() { return __builtin_nanf(""); }
 };
 
 template <class T>
-__device__ __forceinline__ bool fin(T x) {
+__host__ __device__ __forceinline__ bool fin(T x) {
   return __builtin_isfinite(x);
 }
 
 // x/y swap into the +x-normal frame (self-inverse).
 template <class T>
-__device__ __forceinline__ St<T> rot(const St<T>& s, int dir) {
+__host__ __device__ __forceinline__ St<T> rot(const St<T>& s, int dir) {
   if (dir == 0) return s;
   St<T> r = s;
   r.mx = s.my;
@@ -86,12 +86,12 @@ __device__ __forceinline__ St<T> rot(const St<T>& s, int dir) {
 // ---------------------------------------------------------------------------
 // Equation of state.
 template <class T>
-__device__ __forceinline__ T half_sq(T x, T y, T z) {
+__host__ __device__ __forceinline__ T half_sq(T x, T y, T z) {
   return T(0.5) * (x * x + y * y + z * z);
 }
 
 template <class T>
-__device__ __forceinline__ T pressure(const St<T>& u, T gm1) {
+__host__ __device__ __forceinline__ T pressure(const St<T>& u, T gm1) {
   const T kin = half_sq(u.mx, u.my, u.mz) / u.rho;
   const T mag = half_sq(u.bx, u.by, u.bz);
   return gm1 * ((u.en - kin) - mag);
@@ -99,7 +99,7 @@ __device__ __forceinline__ T pressure(const St<T>& u, T gm1) {
 
 // Fast magnetosonic speed along +x for primitive density/pressure.
 template <class T>
-__device__ __forceinline__ T fast_speed_x(T rho, T p, T bx, T by, T bz, T gamma) {
+__host__ __device__ __forceinline__ T fast_speed_x(T rho, T p, T bx, T by, T bz, T gamma) {
   const T inv_rho = T(1) / rho;
   const T a2 = (p > T(0)) ? gamma * p * inv_rho : T(0);
   const T b2 = (bx * bx + by * by + bz * bz) * inv_rho;
@@ -113,18 +113,18 @@ __device__ __forceinline__ T fast_speed_x(T rho, T p, T bx, T by, T bz, T gamma)
 // ---------------------------------------------------------------------------
 // MP limiter (Suresh & Huynh 1997).
 template <class T>
-__device__ __forceinline__ T minmod2(T a, T b) {
+__host__ __device__ __forceinline__ T minmod2(T a, T b) {
   if (a == T(0) || b == T(0) || ((a < T(0)) != (b < T(0)))) return T(0);
   return (__builtin_fabs(a) < __builtin_fabs(b)) ? a : b;
 }
 
 template <class T>
-__device__ __forceinline__ T sgn0(T a) {
+__host__ __device__ __forceinline__ T sgn0(T a) {
   return (a > T(0)) ? T(1) : ((a < T(0)) ? T(-1) : T(0));
 }
 
 template <class T>
-__device__ __forceinline__ T minmod4(T a, T b, T c, T d) {
+__host__ __device__ __forceinline__ T minmod4(T a, T b, T c, T d) {
   const T sa = sgn0(a), sb = sgn0(b), sc = sgn0(c), sd = sgn0(d);
   if (!(sa == sb && sb == sc && sc == sd) || sa == T(0)) return T(0);
   return sa * fmin(fmin(__builtin_fabs(a), __builtin_fabs(b)),
@@ -132,7 +132,7 @@ __device__ __forceinline__ T minmod4(T a, T b, T c, T d) {
 }
 
 template <class T>
-__device__ __forceinline__ T mp_limit(T vl, T vm2, T vm1, T v0, T vp1, T vp2) {
+__host__ __device__ __forceinline__ T mp_limit(T vl, T vm2, T vm1, T v0, T vp1, T vp2) {
   const T d_minus = v0 - vm1;
   const T d_plus = vp1 - v0;
   const T vmp = v0 + minmod2(d_plus, T(4) * d_minus);
@@ -161,12 +161,12 @@ __device__ __forceinline__ T mp_limit(T vl, T vm2, T vm1, T v0, T vp1, T vp2) {
 // Finite-volume left-biased face values: MP5 (2,-13,47,27,-3)/60 and
 // MP7 (-3,25,-101,319,214,-38,4)/420 applied to cell averages.
 template <class T>
-__device__ __forceinline__ T mp5_face(const T* v) {
+__host__ __device__ __forceinline__ T mp5_face(const T* v) {
   const T s = T(2) * v[0] - T(13) * v[1] + T(47) * v[2] + T(27) * v[3] - T(3) * v[4];
   return s * (T(1) / T(60));
 }
 template <class T>
-__device__ __forceinline__ T mp7_face(const T* v) {
+__host__ __device__ __forceinline__ T mp7_face(const T* v) {
   const T s = -T(3) * v[0] + T(25) * v[1] - T(101) * v[2] + T(319) * v[3] + T(214) * v[4] -
               T(38) * v[5] + T(4) * v[6];
   return s * (T(1) / T(420));
@@ -175,6 +175,47 @@ __device__ __forceinline__ T mp7_face(const T* v) {
 // ---------------------------------------------------------------------------
 // Seven-wave eigensystem about a reference state, +x normal, in the
 // conserved 7-vector (rho, mn, mt1, mt2, E, bt1, bt2) (Stone et al. 2008, App. A).
+//
+// Whether the characteristic basis is usable is decided by an absolute
+// 1024-epsilon test on L R - I (include/quasar/numerics/mhd_eigensystem.hpp:
+// 398-431).  Near degenerate states (zero transverse field with a ~ c_ax) the
+// residual sits close to that threshold, so the decision depends on how each
+// operation rounds.  The whole build therefore follows the reference's
+// operation order term by term (its dense M Rp and Lp Minv products reduced to
+// their structurally non-zero terms, in the same summation order) with
+// multiply-add contraction off, so the decision and the vectors are those of
+// the reference arithmetic.
+
+template <class T>
+__host__ __device__ __forceinline__ T scaled_norm3(T x, T y, T z) {
+#pragma clang fp contract(off)
+  const T scale = fmax(__builtin_fabs(x), fmax(__builtin_fabs(y), __builtin_fabs(z)));
+  if (scale == T(0)) return T(0);
+  const T xs = x / scale, ys = y / scale, zs = z / scale;
+  return scale * __builtin_sqrt(xs * xs + ys * ys + zs * zs);
+}
+
+template <class T>
+__host__ __device__ __forceinline__ T half_squared_norm3(T x, T y, T z) {
+#pragma clang fp contract(off)
+  const T scale = fmax(__builtin_fabs(x), fmax(__builtin_fabs(y), __builtin_fabs(z)));
+  if (scale == T(0)) return T(0);
+  const T xs = x / scale, ys = y / scale, zs = z / scale;
+  const T ns = xs * xs + ys * ys + zs * zs;
+  return (T(0.5) * scale) * (scale * ns);
+}
+
+template <class T>
+__host__ __device__ __forceinline__ T kinetic_from_momentum(T mx, T my, T mz, T rho) {
+#pragma clang fp contract(off)
+  const T scale = fmax(__builtin_fabs(mx), fmax(__builtin_fabs(my), __builtin_fabs(mz)));
+  if (scale == T(0)) return T(0);
+  const T xs = mx / scale, ys = my / scale, zs = mz / scale;
+  const T q = scale / __builtin_sqrt(rho);
+  const T ns = xs * xs + ys * ys + zs * zs;
+  return (T(0.5) * q) * (q * ns);
+}
+
 template <class T>
 struct Eig {
   T L[7][7];
@@ -182,67 +223,79 @@ struct Eig {
 };
 
 template <class T>
-__device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<T>& e) {
+__host__ __device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<T>& e) {
+#pragma clang fp contract(off)
+  // Primitive reference state (mhd_state.hpp:183-189, 228-239).
   const T rho = s.rho;
-  const T inv_rho = T(1) / rho;
-  const T vn = s.mx * inv_rho, vt1 = s.my * inv_rho, vt2 = s.mz * inv_rho;
-  const T bn = s.bx, bt1 = s.by, bt2 = s.bz;
-  const T gm1 = gamma - T(1);
-  T p = gm1 * ((s.en - T(0.5) * (vn * s.mx + vt1 * s.my + vt2 * s.mz)) - half_sq(bn, bt1, bt2));
+  const T vn = s.mx / rho, vt1 = s.my / rho, vt2 = s.mz / rho;
+  T p = (gamma - T(1)) * ((s.en - kinetic_from_momentum(s.mx, s.my, s.mz, rho)) - half_squared_norm3(s.bx, s.by, s.bz));
   p = p > T(0) ? p : T(0);
   const T sqrt_rho = __builtin_sqrt(rho);
-  const T inv_sqrt_rho = T(1) / sqrt_rho;
-  const T a2 = gamma * p * inv_rho;
-  const T a = __builtin_sqrt(a2);
-  const T btot2 = (bn * bn + bt1 * bt1 + bt2 * bt2) * inv_rho;
-  const T ca = __builtin_sqrt(btot2);
-  const T cax2 = bn * bn * inv_rho;
-  const T sum = a2 + btot2;
-  T disc = sum * sum - T(4) * a2 * cax2;
-  disc = disc < T(0) ? T(0) : disc;
-  const T cf2 = T(0.5) * (sum + __builtin_sqrt(disc));
-  const T cs2 = cf2 > T(0) ? (a2 * cax2) / cf2 : T(0);
-  const T cf = __builtin_sqrt(cf2);
-  const T cs = __builtin_sqrt(cs2);
+  const T bn = s.bx + T(0), bt1 = s.by + T(0), bt2 = s.bz + T(0);
+  // Magnetosonic speeds through the normalised quadratic (mhd_state.hpp:141-180).
+  T cf, cs, a, ca;
+  {
+    a = (p > T(0) && gamma > T(0)) ? __builtin_sqrt(gamma) * (__builtin_sqrt(p) / sqrt_rho) : T(0);
+    const T ax = bn / sqrt_rho, ay = bt1 / sqrt_rho, az = bt2 / sqrt_rho;
+    ca = scaled_norm3(ax, ay, az);
+    const T cax = __builtin_fabs(ax);
+    const T scale = fmax(a, ca);
+    if (!(scale > T(0))) {
+      cf = T(0);
+      cs = T(0);
+    } else {
+      const T an = a / scale, bnn = ca / scale, can = cax / scale;
+      const T a2n = an * an, b2n = bnn * bnn, can2 = can * can;
+      const T sum = a2n + b2n;
+      T disc = sum * sum - T(4) * a2n * can2;
+      if (disc < T(0)) disc = T(0);
+      const T fast2n = T(0.5) * (sum + __builtin_sqrt(disc));
+      const T slow2n = fast2n > T(0) ? (a2n * can2) / fast2n : T(0);
+      cf = scale * __builtin_sqrt(fast2n);
+      cs = scale * __builtin_sqrt(slow2n);
+    }
+  }
   constexpr T geps = T(64) * Lim<T>::eps;
-  const T rs2 = T(0.70710678118654752440084436210484903928);
+  const T rs2 = T(1) / __builtin_sqrt(T(2));
   T alpha_f, alpha_s;
   {
-    const T denom = cf2 - cs2;
-    if (!(cf > T(0)) || denom <= geps * (cf2 + cs2)) {
+    const T speed_scale = cf > T(0) ? cf : T(1);
+    const T cfn = cf / speed_scale, csn = cs / speed_scale, an = a / speed_scale;
+    const T denom = cfn * cfn - csn * csn;
+    if (denom <= geps * (cfn * cfn + csn * csn)) {
       alpha_f = rs2;
       alpha_s = rs2;
     } else {
-      const T inv_d = T(1) / denom;
-      T af2 = (a2 - cs2) * inv_d;
-      T as2 = (cf2 - a2) * inv_d;
-      af2 = af2 < T(0) ? T(0) : af2;
-      as2 = as2 < T(0) ? T(0) : as2;
+      T af2 = (an * an - csn * csn) / denom;
+      T as2 = (cfn * cfn - an * an) / denom;
+      if (af2 < T(0)) af2 = T(0);
+      if (as2 < T(0)) as2 = T(0);
       alpha_f = __builtin_sqrt(af2);
       alpha_s = __builtin_sqrt(as2);
     }
   }
   T beta1, beta2;
   {
-    const T bt = __builtin_sqrt(bt1 * bt1 + bt2 * bt2);
+    const T bt = scaled_norm3(bt1, bt2, T(0));
+    const T bt_speed = bt / sqrt_rho;
     const T field_scale = a > ca ? a : ca;
-    if (bt == T(0) || bt * inv_sqrt_rho <= geps * (field_scale + Lim<T>::tiny)) {
+    if (bt == T(0) || bt_speed <= geps * (field_scale + Lim<T>::tiny)) {
       beta1 = rs2;
       beta2 = rs2;
     } else {
-      const T ib = T(1) / bt;
-      beta1 = bt1 * ib;
-      beta2 = bt2 * ib;
+      beta1 = bt1 / bt;
+      beta2 = bt2 / bt;
     }
   }
   const T sbn = bn >= T(0) ? T(1) : T(-1);
   const T qf = cf * alpha_f * sbn, qs = cs * alpha_s * sbn;
   const T Af = a * alpha_f * sqrt_rho, As = a * alpha_s * sqrt_rho;
-  const T inv_a2 = a > T(0) ? T(1) / a2 : T(0);
+  const T inv_a2 = a > T(0) ? (T(1) / a) / a : T(0);
   const T Nf = T(0.5) * inv_a2;
   const T rho_a2 = gamma * p;
 
-  // Primitive right eigenvectors Rp[wave][var], var = (rho,vn,vt1,vt2,p,bt1,bt2).
+  // Primitive right eigenvectors Rp[wave][var], var = (rho,vn,vt1,vt2,p,bt1,bt2),
+  // and left eigenvectors Lp[wave][var]; unset entries are zero.
   T Rp[7][7];
   T Lp[7][7];
 #pragma unroll
@@ -267,40 +320,46 @@ __device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<T>& e) {
   Rp[3][0] = T(1);
 
   Lp[0][1] = -Nf * alpha_f * cf; Lp[0][2] = Nf * qs * beta1; Lp[0][3] = Nf * qs * beta2;
-  Lp[0][4] = Nf * alpha_f * inv_rho; Lp[0][5] = Nf * As * beta1 * inv_rho; Lp[0][6] = Nf * As * beta2 * inv_rho;
+  Lp[0][4] = Nf * alpha_f / rho; Lp[0][5] = Nf * As * beta1 / rho; Lp[0][6] = Nf * As * beta2 / rho;
   Lp[6][1] = Nf * alpha_f * cf; Lp[6][2] = -Nf * qs * beta1; Lp[6][3] = -Nf * qs * beta2;
-  Lp[6][4] = Nf * alpha_f * inv_rho; Lp[6][5] = Nf * As * beta1 * inv_rho; Lp[6][6] = Nf * As * beta2 * inv_rho;
+  Lp[6][4] = Nf * alpha_f / rho; Lp[6][5] = Nf * As * beta1 / rho; Lp[6][6] = Nf * As * beta2 / rho;
   Lp[1][2] = -beta2 * rs2; Lp[1][3] = beta1 * rs2;
-  Lp[1][5] = -sbn * beta2 * rs2 * inv_sqrt_rho; Lp[1][6] = sbn * beta1 * rs2 * inv_sqrt_rho;
+  Lp[1][5] = -sbn * beta2 * rs2 / sqrt_rho; Lp[1][6] = sbn * beta1 * rs2 / sqrt_rho;
   Lp[5][2] = -beta2 * rs2; Lp[5][3] = beta1 * rs2;
-  Lp[5][5] = sbn * beta2 * rs2 * inv_sqrt_rho; Lp[5][6] = -sbn * beta1 * rs2 * inv_sqrt_rho;
+  Lp[5][5] = sbn * beta2 * rs2 / sqrt_rho; Lp[5][6] = -sbn * beta1 * rs2 / sqrt_rho;
   Lp[2][1] = -Nf * alpha_s * cs; Lp[2][2] = -Nf * qf * beta1; Lp[2][3] = -Nf * qf * beta2;
-  Lp[2][4] = Nf * alpha_s * inv_rho; Lp[2][5] = -Nf * Af * beta1 * inv_rho; Lp[2][6] = -Nf * Af * beta2 * inv_rho;
+  Lp[2][4] = Nf * alpha_s / rho; Lp[2][5] = -Nf * Af * beta1 / rho; Lp[2][6] = -Nf * Af * beta2 / rho;
   Lp[4][1] = Nf * alpha_s * cs; Lp[4][2] = Nf * qf * beta1; Lp[4][3] = Nf * qf * beta2;
-  Lp[4][4] = Nf * alpha_s * inv_rho; Lp[4][5] = -Nf * Af * beta1 * inv_rho; Lp[4][6] = -Nf * Af * beta2 * inv_rho;
+  Lp[4][4] = Nf * alpha_s / rho; Lp[4][5] = -Nf * Af * beta1 / rho; Lp[4][6] = -Nf * Af * beta2 / rho;
   Lp[3][0] = T(1); Lp[3][4] = -inv_a2;
 
-  // Conserved eigenvectors R = M Rp, L = Lp Minv with M = dU/dW (sparse).
-  const T half_v2 = half_sq(vn, vt1, vt2);
-  const T inv_gm1 = T(1) / gm1;
+  // M = dU/dW and its inverse (mhd_eigensystem.hpp:333-379): only their
+  // non-zero entries enter the products below, in the reference's j order.
+  const T gm1 = gamma - T(1);
+  const T half_v2 = half_squared_norm3(vn, vt1, vt2);
+  const T m41 = rho * vn, m42 = rho * vt1, m43 = rho * vt2, m44 = T(1) / gm1;
+  const T inv_rho = T(1) / rho;
+  const T i10 = -vn * inv_rho, i20 = -vt1 * inv_rho, i30 = -vt2 * inv_rho;
+  const T i40 = gm1 * half_v2, i41 = -gm1 * vn, i42 = -gm1 * vt1, i43 = -gm1 * vt2;
+  const T i45 = -gm1 * bt1, i46 = -gm1 * bt2;
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
-    e.R[0][k] = Rp[k][0];
-    e.R[1][k] = vn * Rp[k][0] + rho * Rp[k][1];
-    e.R[2][k] = vt1 * Rp[k][0] + rho * Rp[k][2];
-    e.R[3][k] = vt2 * Rp[k][0] + rho * Rp[k][3];
-    e.R[4][k] = half_v2 * Rp[k][0] + rho * (vn * Rp[k][1] + vt1 * Rp[k][2] + vt2 * Rp[k][3]) +
-                inv_gm1 * Rp[k][4] + bt1 * Rp[k][5] + bt2 * Rp[k][6];
-    e.R[5][k] = Rp[k][5];
-    e.R[6][k] = Rp[k][6];
-    const T l4 = Lp[k][4] * gm1;
-    e.L[k][0] = Lp[k][0] - (vn * Lp[k][1] + vt1 * Lp[k][2] + vt2 * Lp[k][3]) * inv_rho + l4 * half_v2;
-    e.L[k][1] = Lp[k][1] * inv_rho - l4 * vn;
-    e.L[k][2] = Lp[k][2] * inv_rho - l4 * vt1;
-    e.L[k][3] = Lp[k][3] * inv_rho - l4 * vt2;
-    e.L[k][4] = l4;
-    e.L[k][5] = Lp[k][5] - l4 * bt1;
-    e.L[k][6] = Lp[k][6] - l4 * bt2;
+    const T* r = Rp[k];
+    e.R[0][k] = r[0];
+    e.R[1][k] = vn * r[0] + rho * r[1];
+    e.R[2][k] = vt1 * r[0] + rho * r[2];
+    e.R[3][k] = vt2 * r[0] + rho * r[3];
+    e.R[4][k] = (((((half_v2 * r[0] + m41 * r[1]) + m42 * r[2]) + m43 * r[3]) + m44 * r[4]) + bt1 * r[5]) + bt2 * r[6];
+    e.R[5][k] = r[5];
+    e.R[6][k] = r[6];
+    const T* l = Lp[k];
+    e.L[k][0] = (((l[0] + l[1] * i10) + l[2] * i20) + l[3] * i30) + l[4] * i40;
+    e.L[k][1] = l[1] * inv_rho + l[4] * i41;
+    e.L[k][2] = l[2] * inv_rho + l[4] * i42;
+    e.L[k][3] = l[3] * inv_rho + l[4] * i43;
+    e.L[k][4] = l[4] * gm1;
+    e.L[k][5] = l[4] * i45 + l[5];
+    e.L[k][6] = l[4] * i46 + l[6];
   }
   // Loss of biorthogonality -> component-wise limiting (identity basis).
   bool usable = true;
@@ -308,11 +367,11 @@ __device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<T>& e) {
   for (int i = 0; i < 7; ++i)
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
-      T dot = T(0);
+      T dot = e.L[i][0] * e.R[0][j];
 #pragma unroll
-      for (int k = 0; k < 7; ++k) dot += e.L[i][k] * e.R[k][j];
+      for (int k = 1; k < 7; ++k) dot = dot + e.L[i][k] * e.R[k][j];
       const T expect = (i == j) ? T(1) : T(0);
-      usable = usable && fin(dot) && __builtin_fabs(dot - expect) <= T(1024) * Lim<T>::eps;
+      usable = usable && fin(dot) && !(__builtin_fabs(dot - expect) > T(1024) * Lim<T>::eps);
     }
   if (!usable) {
 #pragma unroll
@@ -328,7 +387,7 @@ __device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<T>& e) {
 // ---------------------------------------------------------------------------
 // Fluxes along +x.
 template <class T>
-__device__ __forceinline__ St<T> phys_flux(const St<T>& u, T gm1) {
+__host__ __device__ __forceinline__ St<T> phys_flux(const St<T>& u, T gm1) {
   const T inv = T(1) / u.rho;
   const T vx = u.mx * inv, vy = u.my * inv, vz = u.mz * inv;
   const T p = pressure(u, gm1);
@@ -348,7 +407,7 @@ __device__ __forceinline__ St<T> phys_flux(const St<T>& u, T gm1) {
 }
 
 template <class T>
-__device__ __forceinline__ St<T> hll_flux(const St<T>& L, const St<T>& R, T sl, T sr, T gm1) {
+__host__ __device__ __forceinline__ St<T> hll_flux(const St<T>& L, const St<T>& R, T sl, T sr, T gm1) {
   const St<T> fl = phys_flux(L, gm1);
   const St<T> fr = phys_flux(R, gm1);
   if (sl >= T(0)) return fl;
@@ -372,7 +431,7 @@ __device__ __forceinline__ St<T> hll_flux(const St<T>& L, const St<T>& R, T sl, 
 }
 
 template <class T>
-__device__ __forceinline__ St<T> lf_flux(const St<T>& L, const St<T>& R, T gamma) {
+__host__ __device__ __forceinline__ St<T> lf_flux(const St<T>& L, const St<T>& R, T gamma) {
   const T gm1 = gamma - T(1);
   const T cfL = fast_speed_x(L.rho, pressure(L, gm1), L.bx, L.by, L.bz, gamma);
   const T cfR = fast_speed_x(R.rho, pressure(R, gm1), R.bx, R.by, R.bz, gamma);
@@ -383,12 +442,12 @@ __device__ __forceinline__ St<T> lf_flux(const St<T>& L, const St<T>& R, T gamma
 }
 
 template <class T>
-__device__ __forceinline__ bool resolved(T value, T scale) {
+__host__ __device__ __forceinline__ bool resolved(T value, T scale) {
   return fin(value) && __builtin_fabs(value) > T(64) * Lim<T>::eps * (scale + Lim<T>::tiny);
 }
 
 template <class T>
-__device__ __noinline__ St<T> hlld_flux(const St<T>& Lin, const St<T>& Rin, T gamma) {
+__host__ __device__ __noinline__ St<T> hlld_flux(const St<T>& Lin, const St<T>& Rin, T gamma) {
   const T gm1 = gamma - T(1);
   if (Lin.rho == Rin.rho && Lin.mx == Rin.mx && Lin.my == Rin.my && Lin.mz == Rin.mz &&
       Lin.en == Rin.en && Lin.bx == Rin.bx && Lin.by == Rin.by && Lin.bz == Rin.bz)
@@ -535,9 +594,6 @@ struct QuadRows {
   double mp5_gauss[3];
   double mp7_point[4][7];
   double mp7_gauss[4];
-  // Outer-ghost one-sided face->cell rows: [width index][relative cell][tap]
-  // for widths 8, 6, 4, 2 (index 0..3).
-  double one_sided[4][8][8];
 };
 
 }  // namespace qmhd

@@ -1,0 +1,562 @@
+// Per-face and per-cell stencil code of the MHD step, shared by the gfx950
+// kernels and by host builds of the same code (compiled with hipcc as host
+// functions) that check the kernel logic on the CPU.
+//
+// Storage: every plane is a padded row-major array, index (i+ng) + pitch*(j+ng),
+// pitch = nx + 2 ng (the reference's Grid2D, include/quasar/core/grid.hpp:84-200).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include "qmhd_kernels.hpp"
+
+namespace qmhd {
+
+__host__ __device__ __forceinline__ std::size_t idx(const GridD& g, int i, int j) {
+  return static_cast<std::size_t>(i + g.ng) + static_cast<std::size_t>(g.pitch) * static_cast<std::size_t>(j + g.ng);
+}
+
+// Periodic wrap or clamp of a face/cell coordinate (mode 4 keeps exchanged guards).
+__host__ __device__ __forceinline__ int face_index(int q, int n, int lo_mode, int hi_mode) {
+  if (lo_mode == 0 && hi_mode == 0) {
+    int w = q % n;
+    return w < 0 ? w + n : w;
+  }
+  if (q < 0) return lo_mode == 4 ? q : 0;
+  if (q > n) return hi_mode == 4 ? q : n;
+  return q;
+}
+
+// ---------------------------------------------------------------------------
+// Face -> cell collocation of the staggered normal field.
+
+__host__ __device__ __forceinline__ int colloc_width(int extent, int ng) {
+  const int req = ng >= 4 ? 8 : ng >= 3 ? 6 : ng >= 2 ? 4 : 2;
+  if (req >= 8 && extent >= 8) return 8;
+  if (req >= 6 && extent >= 6) return 6;
+  if (req >= 4 && extent >= 4) return 4;
+  return extent >= 2 ? 2 : 1;
+}
+
+// Sample of the face array along `axis` at coordinate q on line `fixed`.
+__host__ __device__ __forceinline__ double axis_face(const GridD& g, const double* f, int axis, int q, int fixed) {
+  return axis == 0 ? f[idx(g, q, fixed)] : f[idx(g, fixed, q)];
+}
+
+// Correctly rounded (half-even) sum of up to N binary64 terms; zero terms are
+// skipped and an exactly cancelling sum is +0.  The terms are accumulated into
+// a non-overlapping expansion of increasing magnitude (Shewchuk's
+// grow-expansion with the swap-ordered fast two-sum), kept in registers by full
+// unrolling, and rounded from the top with the half-even correction of the
+// published fsum finalisation.  This is the value the reference's exact
+// expansion arithmetic returns (include/quasar/numerics/scaled_arithmetic.hpp:
+// 121-208).  Callers pass finite terms.
+template <int N>
+__host__ __device__ __forceinline__ double cr_sum(const double* t) {
+#pragma clang fp contract(off)
+  double p[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) p[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double x = t[k];
+    if (x != 0.0) {
+#pragma unroll
+      for (int q = 0; q < k; ++q) {
+        const double y = p[q];
+        const bool sw = __builtin_fabs(x) < __builtin_fabs(y);
+        const double big = sw ? y : x, small = sw ? x : y;
+        const double hi = big + small;
+        p[q] = small - (hi - big);
+        x = hi;
+      }
+      p[k] = x;
+    }
+  }
+  double hi = 0.0, lo = 0.0, next = 0.0;
+  bool broke = false, have_next = false;
+#pragma unroll
+  for (int q = N - 1; q >= 0; --q) {
+    const double y = p[q];
+    if (!broke) {
+      const double x = hi;
+      hi = x + y;
+      lo = y - (hi - x);
+      broke = lo != 0.0;
+    } else if (!have_next && y != 0.0) {
+      next = y;
+      have_next = true;
+    }
+  }
+  if (broke && have_next && ((lo < 0.0 && next < 0.0) || (lo > 0.0 && next > 0.0))) {
+    const double y = lo * 2.0;
+    const double x = hi + y;
+    if (y == x - hi) hi = x;
+  }
+  return hi == 0.0 ? 0.0 : hi;
+}
+
+// Integer-weight stencil sum_k w_k s_k / D with the numerator correctly rounded
+// from the error-free products (mhd_staggering.hpp:59-70): the value of the
+// reference's exact rational collocation.
+template <int W>
+__host__ __device__ __forceinline__ double rational_stencil(const double* w, const double* s, double denom) {
+#pragma clang fp contract(off)
+  bool finite = true;
+#pragma unroll
+  for (int k = 0; k < W; ++k) finite = finite && __builtin_isfinite(s[k]);
+  if (!finite) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) acc = acc + w[k] * s[k];
+    return acc / denom;
+  }
+  double t[2 * W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const double hi = w[k] * s[k];
+    t[2 * k] = hi;
+    t[2 * k + 1] = fma(w[k], s[k], -hi);
+  }
+  return cr_sum<2 * W>(t) / denom;
+}
+
+// Correctly rounded sum as cr_sum<N>, with the expansion compacted in a loop
+// (fsum's partials); for the rarely taken paths, where a small register
+// footprint matters more than speed.
+__host__ __device__ __noinline__ inline double cr_sum_loop(const double* t, int n) {
+#pragma clang fp contract(off)
+  double partials[40];
+  int np = 0;
+#pragma unroll 1
+  for (int k = 0; k < n; ++k) {
+    double x = t[k];
+    if (x == 0.0) continue;
+    int i = 0;
+#pragma unroll 1
+    for (int q = 0; q < np; ++q) {
+      double y = partials[q];
+      if (__builtin_fabs(x) < __builtin_fabs(y)) {
+        const double tmp = x;
+        x = y;
+        y = tmp;
+      }
+      const double hi = x + y;
+      const double lo = y - (hi - x);
+      if (lo != 0.0) partials[i++] = lo;
+      x = hi;
+    }
+    np = i;
+    partials[np++] = x;
+  }
+  if (np == 0) return 0.0;
+  int left = np;
+  double hi = partials[--left];
+  double lo = 0.0;
+#pragma unroll 1
+  while (left > 0) {
+    const double x = hi;
+    const double y = partials[--left];
+    hi = x + y;
+    lo = y - (hi - x);
+    if (lo != 0.0) break;
+  }
+  if (left > 0 && ((lo < 0.0 && partials[left - 1] < 0.0) || (lo > 0.0 && partials[left - 1] > 0.0))) {
+    const double y = lo * 2.0;
+    const double x = hi + y;
+    if (y == x - hi) hi = x;
+  }
+  return hi == 0.0 ? 0.0 : hi;
+}
+
+// One-sided closure of the outer ghost layers (mhd_staggering.hpp:111-173): the
+// 4-point Gauss integral over cell `rel` of the barycentric interpolant through
+// `width` consecutive faces, as a correctly rounded sum of the rounded
+// product-quotient terms ((w_q b_k) s_k / (x_q - k)) / den_q, den_q the
+// correctly rounded sum of b_k / (x_q - k).
+__host__ __device__ __noinline__ inline double one_sided_average(const double* s, int width, int rel) {
+#pragma clang fp contract(off)
+  const double xq[4] = {0.0694318442029737124, 0.3300094782075718676, 0.6699905217924281324,
+                        0.9305681557970262876};
+  const double wq[4] = {0.1739274225687269287, 0.3260725774312730713, 0.3260725774312730713,
+                        0.1739274225687269287};
+  double bary[8];
+  double c = 1.0;  // (-1)^k C(width-1, k), exact small integers
+#pragma unroll 1
+  for (int k = 0; k < width; ++k) {
+    bary[k] = (k % 2 == 0) ? c : -c;
+    c = c * double(width - 1 - k) / double(k + 1);
+  }
+  bool finite = true;
+#pragma unroll 1
+  for (int k = 0; k < width; ++k) finite = finite && __builtin_isfinite(s[k]);
+  double terms[32];
+  int n = 0;
+#pragma unroll 1
+  for (int q = 0; q < 4; ++q) {
+    const double x = double(rel) + xq[q];
+    double dt[8];
+#pragma unroll 1
+    for (int k = 0; k < width; ++k) dt[k] = bary[k] / (x - double(k));
+    const double den = cr_sum_loop(dt, width);
+    const double wscaled = wq[q] / 1.0;
+#pragma unroll 1
+    for (int k = 0; k < width; ++k) terms[n++] = (((wscaled * bary[k]) * s[k]) / (x - double(k))) / den;
+  }
+  if (!finite) {
+    double acc = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < n; ++k) acc = acc + terms[k];
+    return acc;
+  }
+  return cr_sum_loop(terms, n);
+}
+
+// High-order cell average (8/6/4/2 taps, one-sided closures in the outer
+// ghosts), mhd_staggering.hpp:175-236.
+__host__ __device__ inline double face_to_cell(const GridD& g, const double* f, int axis, int cell, int fixed) {
+  const int n = axis == 0 ? g.nx : g.ny;
+  const int width = colloc_width(n + 2 * g.ng, g.ng);
+  const int lo = -g.ng, hi = n + g.ng - 1;
+  if (width == 1) return axis_face(g, f, axis, cell, fixed);
+  const int start_c = cell - (width / 2 - 1);
+  double s[8];
+  if (start_c >= lo && start_c + width - 1 <= hi) {
+    for (int k = 0; k < width; ++k) s[k] = axis_face(g, f, axis, start_c + k, fixed);
+    if (width == 8) {
+      const double w[8] = {-191.0, 1879.0, -9531.0, 68323.0, 68323.0, -9531.0, 1879.0, -191.0};
+      return rational_stencil<8>(w, s, 120960.0);
+    }
+    if (width == 6) {
+      const double w[6] = {11.0, -93.0, 802.0, 802.0, -93.0, 11.0};
+      return rational_stencil<6>(w, s, 1440.0);
+    }
+    if (width == 4) {
+      const double w[4] = {-1.0, 13.0, 13.0, -1.0};
+      return rational_stencil<4>(w, s, 24.0);
+    }
+    const double w[2] = {1.0, 1.0};
+    return rational_stencil<2>(w, s, 2.0);
+  }
+  int start = start_c < lo ? lo : start_c;
+  if (start + width - 1 > hi) start = hi - width + 1;
+  for (int k = 0; k < width; ++k) s[k] = axis_face(g, f, axis, start + k, fixed);
+  return one_sided_average(s, width, cell - start);
+}
+
+// First-order (two bounding faces) cell average used by the low-order retry.
+__host__ __device__ __forceinline__ double face_to_cell_lo(const GridD& g, const double* f, int axis, int cell, int fixed) {
+  const int n = axis == 0 ? g.nx : g.ny;
+  const double a = axis_face(g, f, axis, cell, fixed);
+  if (cell >= n + g.ng - 1) return a;
+  const double b = axis_face(g, f, axis, cell + 1, fixed);
+  const double s[2] = {a, b};
+  const double w[2] = {1.0, 1.0};
+  return rational_stencil<2>(w, s, 2.0);
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction.
+
+template <class T>
+__host__ __device__ __forceinline__ St<T> load_cell(const GridD& g, const StateView& u, const double* bxc,
+                                           const double* byc, int i, int j) {
+  const std::size_t k = idx(g, i, j);
+  return St<T>{T(u.c[0][k]), T(u.c[1][k]), T(u.c[2][k]), T(u.c[3][k]),
+               T(u.c[4][k]), T(bxc[k]),    T(byc[k]),    T(u.c[7][k])};
+}
+
+template <class T>
+__host__ __device__ __forceinline__ St<T> load_cell_lo(const GridD& g, const StateView& u, int i, int j) {
+  const std::size_t k = idx(g, i, j);
+  return St<T>{T(u.c[0][k]), T(u.c[1][k]), T(u.c[2][k]), T(u.c[3][k]), T(u.c[4][k]),
+               T(face_to_cell_lo(g, u.c[5], 0, i, j)), T(face_to_cell_lo(g, u.c[6], 1, j, i)), T(u.c[7][k])};
+}
+
+template <class T>
+__host__ __device__ __forceinline__ bool admissible_state(const St<T>& s, T gm1) {
+  if (!(fin(s.rho) && s.rho > T(0))) return false;
+  const T inv = T(1) / s.rho;
+  const T vx = s.mx * inv, vy = s.my * inv, vz = s.mz * inv;
+  const T p = pressure(s, gm1);
+  return fin(vx) && fin(vy) && fin(vz) && fin(p) && p > T(0) && fin(s.mx) && fin(s.my) && fin(s.mz) &&
+         fin(s.en) && fin(s.bx) && fin(s.by) && fin(s.bz);
+}
+
+template <class T>
+__host__ __device__ __forceinline__ void set_normal_b(St<T>& s, int dir, T v) {
+  T& bn = dir == 0 ? s.bx : s.by;
+  s.en = fma(-T(0.5) * bn, bn, fma(T(0.5) * v, v, s.en));
+  bn = v;
+}
+
+template <class T>
+__host__ __device__ __forceinline__ T limited_slope(T back, T fwd, int drop) {
+  if (drop < 0) return fwd;
+  if (drop > 0) return back;
+  return minmod2(back, fwd);
+}
+
+__host__ __device__ __forceinline__ int slope_drop(int c, int n, int lo, int hi) {
+  if (lo != 0 && lo != 4 && c <= 0) return -1;
+  if (hi != 0 && hi != 4 && c >= n - 1) return +1;
+  return 0;
+}
+
+// Second-order MUSCL-minmod in primitive variables (also the MP fallback).
+template <class T>
+__host__ __device__ __noinline__ void muscl_face(const GridD& g, int dir, int i, int j, T gamma, int side_lo, int side_hi,
+                                        const StateView& u, const double* bxc, const double* byc, St<T>& ul,
+                                        St<T>& ur) {
+  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+  const T gm1 = gamma - T(1);
+  St<T> c[4];
+  for (int s = 0; s < 4; ++s) c[s] = load_cell<T>(g, u, bxc, byc, i + (s - 2) * di, j + (s - 2) * dj);
+  T w[4][8];
+  for (int s = 0; s < 4; ++s) {
+    const T inv = T(1) / c[s].rho;
+    w[s][0] = c[s].rho;
+    w[s][1] = c[s].mx * inv;
+    w[s][2] = c[s].my * inv;
+    w[s][3] = c[s].mz * inv;
+    w[s][4] = pressure(c[s], gm1);
+    w[s][5] = c[s].bx;
+    w[s][6] = c[s].by;
+    w[s][7] = c[s].bz;
+  }
+  const int n_dir = dir == 0 ? g.nx : g.ny;
+  const int drop_m = slope_drop((dir == 0 ? i : j) - 1, n_dir, side_lo, side_hi);
+  const int drop_p = slope_drop(dir == 0 ? i : j, n_dir, side_lo, side_hi);
+  T wl[8], wr[8];
+  for (int q = 0; q < 8; ++q) {
+    wl[q] = w[1][q] + T(0.5) * limited_slope(w[1][q] - w[0][q], w[2][q] - w[1][q], drop_m);
+    wr[q] = w[2][q] - T(0.5) * limited_slope(w[2][q] - w[1][q], w[3][q] - w[2][q], drop_p);
+  }
+  auto to_cons = [&](const T* p) {
+    St<T> s;
+    s.rho = p[0];
+    s.mx = p[0] * p[1];
+    s.my = p[0] * p[2];
+    s.mz = p[0] * p[3];
+    s.bx = p[5];
+    s.by = p[6];
+    s.bz = p[7];
+    s.en = p[4] / gm1 + p[0] * half_sq(p[1], p[2], p[3]) + half_sq(p[5], p[6], p[7]);
+    return s;
+  };
+  ul = to_cons(wl);
+  ur = to_cons(wr);
+}
+
+template <class T, int ORDER>
+__host__ __device__ __forceinline__ void mp_char_face(const GridD& g, int dir, int i, int j, T gamma, const StateView& u,
+                                             const double* bxc, const double* byc, St<T>& ul, St<T>& ur) {
+  constexpr int H = ORDER == 5 ? 2 : 3;
+  constexpr int UW = 2 * H + 2;
+  constexpr int W = 2 * H + 1;
+  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+  St<T> cells[UW];
+#pragma unroll
+  for (int q = 0; q < UW; ++q) cells[q] = rot(load_cell<T>(g, u, bxc, byc, i + (q - 1 - H) * di, j + (q - 1 - H) * dj), dir);
+  St<T> ref;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) set_comp(ref, c, T(0.5) * comp(cells[H], c) + T(0.5) * comp(cells[H + 1], c));
+  Eig<T> e;
+  eig_build(ref, gamma, e);
+  // Deltas in the 7-vector (rho, mn, mt1, mt2, E, bt1, bt2).
+  T d[UW][7];
+#pragma unroll
+  for (int q = 0; q < UW; ++q) {
+    d[q][0] = cells[q].rho - ref.rho;
+    d[q][1] = cells[q].mx - ref.mx;
+    d[q][2] = cells[q].my - ref.my;
+    d[q][3] = cells[q].mz - ref.mz;
+    d[q][4] = cells[q].en - ref.en;
+    d[q][5] = cells[q].by - ref.by;
+    d[q][6] = cells[q].bz - ref.bz;
+  }
+  T aL[7], aR[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    aL[v] = T(0);
+    aR[v] = T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    T w[UW];
+#pragma unroll
+    for (int q = 0; q < UW; ++q) {
+      T acc = T(0);
+#pragma unroll
+      for (int v = 0; v < 7; ++v) acc += e.L[k][v] * d[q][v];
+      w[q] = acc;
+    }
+    // Left state: cells 0..W-1 (centre H), right state: reversed cells 1..W.
+    T sl[W], sr[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+      sl[s] = w[s];
+      sr[s] = w[W - s];
+    }
+    const T fl = ORDER == 5 ? mp5_face(sl) : mp7_face(sl);
+    const T fr = ORDER == 5 ? mp5_face(sr) : mp7_face(sr);
+    constexpr int o = ORDER == 5 ? 0 : 1;
+    const T wl = mp_limit(fl, sl[o], sl[o + 1], sl[o + 2], sl[o + 3], sl[o + 4]);
+    const T wr = mp_limit(fr, sr[o], sr[o + 1], sr[o + 2], sr[o + 3], sr[o + 4]);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      aL[v] += e.R[v][k] * wl;
+      aR[v] += e.R[v][k] * wr;
+    }
+  }
+  St<T> sL{ref.rho + aL[0], ref.mx + aL[1], ref.my + aL[2], ref.mz + aL[3], ref.en + aL[4], ref.bx + T(0), ref.by + aL[5], ref.bz + aL[6]};
+  St<T> sR{ref.rho + aR[0], ref.mx + aR[1], ref.my + aR[2], ref.mz + aR[3], ref.en + aR[4], ref.bx + T(0), ref.by + aR[5], ref.bz + aR[6]};
+  ul = rot(sL, dir);
+  ur = rot(sR, dir);
+}
+
+// Interface states of one face normal to `dir` at the scheme's order with the
+// reference's fallback chain (mhd_reconstruct.hip:465-670): characteristic MP
+// -> MUSCL when an MP state is inadmissible; after the normal field is set,
+// piecewise-constant cell states, then a NaN sentinel.
+template <class T, int ORDER>
+__host__ __device__ __forceinline__ void reconstruct_face(const GridD& g, int dir, int i, int j, T gamma, int side_lo,
+                                                          int side_hi, const StateView& u, const double* bxc,
+                                                          const double* byc, St<T>& ul, St<T>& ur) {
+  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+  const T gm1 = gamma - T(1);
+  if (ORDER <= 1) {
+    ul = load_cell_lo<T>(g, u, i - di, j - dj);
+    ur = load_cell_lo<T>(g, u, i, j);
+  } else if (ORDER == 2) {
+    muscl_face<T>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+  } else {
+    mp_char_face<T, ORDER>(g, dir, i, j, gamma, u, bxc, byc, ul, ur);
+    if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1))
+      muscl_face<T>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+  }
+  const T bn = T((dir == 0 ? u.c[5] : u.c[6])[idx(g, i, j)]);
+  set_normal_b(ul, dir, bn);
+  set_normal_b(ur, dir, bn);
+  if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1)) {
+    if (ORDER <= 1) {
+      ul = load_cell_lo<T>(g, u, i - di, j - dj);
+      ur = load_cell_lo<T>(g, u, i, j);
+    } else {
+      ul = load_cell<T>(g, u, bxc, byc, i - di, j - dj);
+      ur = load_cell<T>(g, u, bxc, byc, i, j);
+    }
+    set_normal_b(ul, dir, bn);
+    set_normal_b(ur, dir, bn);
+  }
+  if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1)) {
+    const T nan = Lim<T>::nan();
+    ul = St<T>{nan, nan, nan, nan, nan, nan, nan, nan};
+    ur = ul;
+  }
+}
+
+// Cell E_z = vy Bx - vx By in the reference's arithmetic (mhd_ct.hip:414-420):
+// velocities by division and two rounded products added once.  The corner's
+// one-dimensional shortcuts test cell E_z values for exact equality, so these
+// values must round as the reference's do.
+template <class T>
+__host__ __device__ __forceinline__ T physical_ez(T rho, T mx, T my, T bx, T by) {
+#pragma clang fp contract(off)
+  const T vx = mx / rho;
+  const T vy = my / rho;
+  return vy * bx + vx * (-by);
+}
+
+// ---------------------------------------------------------------------------
+// CT corner EMF at corner (i, j): E_z = ex + ey - e_c, with ex/ey the
+// face-average -> edge interpolations of the directional Godunov EMFs
+// (-F_x(By), +F_y(Bx)) and e_c the cell-average -> corner tensor interpolation
+// of the cell E_z, plus the exact one-dimensional shortcuts
+// (mhd_ct.hip:811-878).
+
+__host__ __device__ __forceinline__ double edge_interp(int order, const double* e) {
+  if (order >= 7) {
+    bool constant = true;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) constant = constant && e[k] == e[0];
+    if (constant) return e[0];
+    return (-3.0 * e[0] + 29.0 * e[1] - 139.0 * e[2] + 533.0 * e[3] + 533.0 * e[4] - 139.0 * e[5] + 29.0 * e[6] -
+            3.0 * e[7]) / 840.0;
+  }
+  if (order >= 5) {
+    bool constant = true;
+#pragma unroll
+    for (int k = 1; k < 6; ++k) constant = constant && e[k] == e[0];
+    if (constant) return e[0];
+    return (e[0] - 8.0 * e[1] + 37.0 * e[2] + 37.0 * e[3] - 8.0 * e[4] + e[5]) / 60.0;
+  }
+  return e[0] == e[1] ? e[0] : (e[0] + e[1]) / 2.0;
+}
+
+template <class T>
+__host__ __device__ inline double corner_ez(const GridD& g, int order, const Flags4& m, const FluxView<T>& fx,
+                                            const FluxView<T>& fy, const double* cell_ez, const int* eqx,
+                                            const int* eqy, int i, int j) {
+  const bool conducting = (i == 0 && (m.side[0] == 2 || m.side[0] == 3)) || (i == g.nx && m.side[1] == 2) ||
+                          (j == 0 && m.side[2] == 2) || (j == g.ny && m.side[3] == 2);
+  if (conducting) return 0.0;
+  const int lo = order >= 7 ? -4 : order >= 5 ? -3 : -1;
+  const int nt = order >= 7 ? 8 : order >= 5 ? 6 : 2;
+  const int rh = order >= 7 ? 3 : order >= 5 ? 2 : 0;  // transverse recovery half-width
+  auto mi = [&](int q) { return face_index(q, g.nx, m.side[0], m.side[1]); };
+  auto mj = [&](int q) { return face_index(q, g.ny, m.side[2], m.side[3]); };
+  double e[8];
+  for (int t = 0; t < nt; ++t) e[t] = -static_cast<double>(fx.c[6][idx(g, mi(i), mj(j + lo + t))]);
+  const double ex = edge_interp(order, e);
+  for (int t = 0; t < nt; ++t) e[t] = static_cast<double>(fy.c[5][idx(g, mi(i + lo + t), mj(j))]);
+  const double ey = edge_interp(order, e);
+  double line[8];
+  for (int q = 0; q < nt; ++q) {
+    for (int t = 0; t < nt; ++t) e[t] = cell_ez[idx(g, mi(i + lo + t), mj(j + lo + q))];
+    line[q] = edge_interp(order, e);
+  }
+  const double centered = edge_interp(order, line);
+  // x-only data: every y-face problem in the stencil has no jump and the cell
+  // E_z is constant along y; y-only symmetrically.
+  // Face `t` of the interpolation stencil is mapped into the face table first;
+  // its point-recovery problem then spans offsets -rh..rh about that mapped
+  // face, mapped again (periodic wrap or physical clamp).
+  auto no_jump_line = [&](const int* eq, int d) {
+    for (int t = 0; t < nt; ++t) {
+      const int fi = d == 0 ? mi(i) : mi(i + lo + t);
+      const int fj = d == 0 ? mj(j + lo + t) : mj(j);
+      for (int off = -rh; off <= rh; ++off) {
+        const int ii = d == 0 ? mi(fi) : mi(fi + off);
+        const int jj = d == 0 ? mj(fj + off) : mj(fj);
+        if (ii < -g.ng || ii >= g.nx + g.ng || jj < -g.ng || jj >= g.ny + g.ng || eq[idx(g, ii, jj)] == 0)
+          return false;
+      }
+    }
+    return true;
+  };
+  auto constant_along = [&](int axis) {
+    for (int v = 0; v < nt; ++v) {
+      const double ref = axis == 0 ? cell_ez[idx(g, mi(i + lo), mj(j + lo + v))] : cell_ez[idx(g, mi(i + lo + v), mj(j + lo))];
+      for (int w = 1; w < nt; ++w) {
+        const double c = axis == 0 ? cell_ez[idx(g, mi(i + lo + w), mj(j + lo + v))]
+                                   : cell_ez[idx(g, mi(i + lo + v), mj(j + lo + w))];
+        if (c != ref) return false;
+      }
+    }
+    return true;
+  };
+  const bool x_only = no_jump_line(eqy, 1) && constant_along(1);
+  const bool y_only = no_jump_line(eqx, 0) && constant_along(0);
+  double r;
+  if (x_only && !y_only)
+    r = ex;
+  else if (y_only && !x_only)
+    r = ey;
+  else
+    r = (ex + ey) - centered;
+  return r;
+}
+
+}  // namespace qmhd

@@ -1,0 +1,141 @@
+// Host build of the gfx950 kernels' per-face stencil code (quasar_amd/csrc/
+// qmhd_stencil.hpp), compiled with hipcc for the host only.  Test
+// infrastructure: it runs the exact functions the kernels call, face by face
+// on the CPU, so the kernel logic (branches, fallbacks, index maps) can be
+// compared with the oracle without a GPU.
+#include <cstring>
+
+#include "../../quasar_amd/csrc/qmhd_stencil.hpp"
+
+using namespace qmhd;
+
+extern "C" {
+
+struct HkGrid {
+  int32_t nx, ny, ng, pad;
+  double lx, ly, ox, oy, cell_dx, cell_dy;
+};
+
+static GridD grid_of(const HkGrid* h) {
+  GridD g;
+  g.nx = h->nx;
+  g.ny = h->ny;
+  g.ng = h->ng;
+  g.pitch = h->nx + 2 * h->ng;
+  g.dx = h->cell_dx > 0 ? h->cell_dx : h->lx / h->nx;
+  g.dy = h->cell_dy > 0 ? h->cell_dy : h->ly / h->ny;
+  g.ox = h->ox;
+  g.oy = h->oy;
+  g.lx = h->lx;
+  g.ly = h->ly;
+  return g;
+}
+
+// Face -> cell collocation over every padded cell (the prep kernel's loop).
+void hk_collocate(const HkGrid* hg, int axis, int co, const double* face, double* cell) {
+  const GridD g = grid_of(hg);
+  for (int j = -g.ng; j < g.ny + g.ng; ++j)
+    for (int i = -g.ng; i < g.nx + g.ng; ++i) {
+      double v;
+      if (axis == 0)
+        v = co == 1 ? face_to_cell_lo(g, face, 0, i, j) : face_to_cell(g, face, 0, i, j);
+      else
+        v = co == 1 ? face_to_cell_lo(g, face, 1, j, i) : face_to_cell(g, face, 1, j, i);
+      cell[idx(g, i, j)] = v;
+    }
+}
+
+// Interface states of every face the reconstruction kernel covers, from the
+// collocated cell fields bxc/byc (every padded cell).
+void hk_reconstruct_from(const HkGrid* hg, int dir, int order, const int32_t* flags4, double gamma,
+                         const double* const* u8, const double* bxc, const double* byc, double* const* out16) {
+  const GridD g = grid_of(hg);
+  StateView u;
+  for (int c = 0; c < 8; ++c) u.c[c] = u8[c];
+  const int side_lo = dir == 0 ? flags4[0] : flags4[2];
+  const int side_hi = dir == 0 ? flags4[1] : flags4[3];
+  const int i0 = dir == 0 ? 0 : -g.ng, i1 = dir == 0 ? g.nx : g.nx + g.ng - 1;
+  const int j0 = dir == 0 ? -g.ng : 0, j1 = dir == 0 ? g.ny + g.ng - 1 : g.ny;
+  for (int j = j0; j <= j1; ++j)
+    for (int i = i0; i <= i1; ++i) {
+      St<double> ul, ur;
+      if (order == 7)
+        reconstruct_face<double, 7>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+      else if (order == 5)
+        reconstruct_face<double, 5>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+      else if (order == 2)
+        reconstruct_face<double, 2>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+      else
+        reconstruct_face<double, 1>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+      const std::size_t k = idx(g, i, j);
+      for (int c = 0; c < 8; ++c) {
+        out16[c][k] = comp(ul, c);
+        out16[8 + c][k] = comp(ur, c);
+      }
+    }
+}
+
+// As above with the kernels' own collocation.
+void hk_reconstruct(const HkGrid* hg, int dir, int order, const int32_t* flags4, double gamma,
+                    const double* const* u8, double* const* out16) {
+  const GridD g = grid_of(hg);
+  const std::size_t n = static_cast<std::size_t>(g.pitch) * (g.ny + 2 * g.ng);
+  double* bxc = new double[n];
+  double* byc = new double[n];
+  hk_collocate(hg, 0, 0, u8[5], bxc);
+  hk_collocate(hg, 1, 0, u8[6], byc);
+  hk_reconstruct_from(hg, dir, order, flags4, gamma, u8, bxc, byc, out16);
+  delete[] bxc;
+  delete[] byc;
+}
+
+// The characteristic system at one state, for inspection.
+int hk_eigen(const double* s8, double gamma, double* L49, double* R49) {
+  St<double> s{s8[0], s8[1], s8[2], s8[3], s8[4], s8[5], s8[6], s8[7]};
+  Eig<double> e;
+  eig_build(s, gamma, e);
+  for (int i = 0; i < 7; ++i)
+    for (int j = 0; j < 7; ++j) {
+      L49[i * 7 + j] = e.L[i][j];
+      R49[i * 7 + j] = e.R[i][j];
+    }
+  return 0;
+}
+
+// One HLLD flux along +x.
+void hk_hlld(const double* l8, const double* r8, double gamma, double* f8) {
+  St<double> L{l8[0], l8[1], l8[2], l8[3], l8[4], l8[5], l8[6], l8[7]};
+  St<double> R{r8[0], r8[1], r8[2], r8[3], r8[4], r8[5], r8[6], r8[7]};
+  const St<double> f = hlld_flux(L, R, gamma);
+  for (int c = 0; c < 8; ++c) f8[c] = comp(f, c);
+}
+
+// Low-order cell E_z (the MUSCL / piecewise-constant path of the cell-EMF
+// kernel) on cells [0,nx] x [0,ny] from the collocated fields.
+void hk_cell_ez_lo(const HkGrid* hg, const double* const* u8, const double* bxc, const double* byc,
+                   double* cell_ez) {
+  const GridD g = grid_of(hg);
+  for (int j = 0; j <= g.ny; ++j)
+    for (int i = 0; i <= g.nx; ++i) {
+      const std::size_t k = idx(g, i, j);
+      cell_ez[k] = physical_ez(u8[0][k], u8[1][k], u8[2][k], bxc[k], byc[k]);
+    }
+}
+
+// Corner E_z on corners [0,nx] x [0,ny].
+void hk_corner(const HkGrid* hg, int order, const int32_t* flags4, const double* const* fx8,
+               const double* const* fy8, const double* cell_ez, const int32_t* eqx, const int32_t* eqy,
+               double* ez) {
+  const GridD g = grid_of(hg);
+  Flags4 m;
+  for (int s = 0; s < 4; ++s) m.side[s] = flags4[s];
+  FluxView<double> fx, fy;
+  for (int c = 0; c < 8; ++c) {
+    fx.c[c] = const_cast<double*>(fx8[c]);
+    fy.c[c] = const_cast<double*>(fy8[c]);
+  }
+  for (int j = 0; j <= g.ny; ++j)
+    for (int i = 0; i <= g.nx; ++i) ez[idx(g, i, j)] = corner_ez<double>(g, order, m, fx, fy, cell_ez, eqx, eqy, i, j);
+}
+
+}  // extern "C"

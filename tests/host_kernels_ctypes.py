@@ -1,0 +1,74 @@
+"""ctypes bindings to the host build of the kernels' stencil code
+(tests/host_kernels/_build/libhost_kernels.so): the same functions the gfx950
+kernels call, run face by face on the CPU."""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+from oracle_ctypes import OrcGrid, _flags, _planes
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+HOST_SO = ROOT / "tests" / "host_kernels" / "_build" / "libhost_kernels.so"
+
+
+def ensure_built() -> None:
+    if not HOST_SO.exists():
+        subprocess.run(["make", "-C", str(HOST_SO.parents[1])], check=True, stdout=subprocess.DEVNULL)
+
+
+class HostKernels:
+    def __init__(self):
+        ensure_built()
+        L = C.CDLL(str(HOST_SO))
+        L.hk_collocate.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.hk_reconstruct.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_int, C.c_void_p, C.c_double,
+                                     C.c_void_p, C.c_void_p]
+        L.hk_reconstruct_from.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_int, C.c_void_p, C.c_double,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hk_eigen.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+        L.hk_hlld.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+        L.hk_cell_ez_lo.argtypes = [C.POINTER(OrcGrid), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hk_corner.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p, C.c_void_p]
+        self.lib = L
+
+    def collocate(self, g, axis, co, face):
+        out = np.zeros_like(face)
+        self.lib.hk_collocate(C.byref(g), axis, co, face.ctypes.data, out.ctypes.data)
+        return out
+
+    def reconstruct(self, g, d, order, flags, gamma, u8):
+        out = [np.zeros_like(u8[0]) for _ in range(16)]
+        self.lib.hk_reconstruct(C.byref(g), d, order, _flags(flags), gamma, _planes(u8), _planes(out))
+        return out
+
+    def eigen(self, s8, gamma):
+        s = np.ascontiguousarray(s8, dtype=np.float64)
+        L = np.zeros(49)
+        R = np.zeros(49)
+        self.lib.hk_eigen(s.ctypes.data, gamma, L.ctypes.data, R.ctypes.data)
+        return L.reshape(7, 7), R.reshape(7, 7)
+
+    def hlld(self, l8, r8, gamma):
+        lv = np.ascontiguousarray(l8, dtype=np.float64)
+        rv = np.ascontiguousarray(r8, dtype=np.float64)
+        f = np.zeros(8)
+        self.lib.hk_hlld(lv.ctypes.data, rv.ctypes.data, gamma, f.ctypes.data)
+        return f
+
+    def cell_ez_lo(self, g, u8, bxc, byc):
+        out = np.zeros_like(u8[0])
+        self.lib.hk_cell_ez_lo(C.byref(g), _planes(u8), bxc.ctypes.data, byc.ctypes.data, out.ctypes.data)
+        return out
+
+    def corner(self, g, order, flags, fx8, fy8, cell_ez, eqx, eqy):
+        out = np.zeros_like(cell_ez)
+        ex = np.ascontiguousarray(eqx, dtype=np.int32)
+        ey = np.ascontiguousarray(eqy, dtype=np.int32)
+        self.lib.hk_corner(C.byref(g), order, _flags(flags), _planes(fx8), _planes(fy8), cell_ez.ctypes.data,
+                           ex.ctypes.data, ey.ctypes.data, out.ctypes.data)
+        return out
