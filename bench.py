@@ -34,28 +34,37 @@ sys.path.insert(0, str(ROOT))
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 
-# Algorithmic cost per unit of each launch kind, FP64.  A unit is one face of
-# the launch's direction for reconstruct/flux (the launch covers (n+1) faces
-# per row over the interior rows), one interior cell for the others.
+# Algorithmic cost per unit of each launch kind, FP64 MP7.  Units per launch
+# are given by `units()`: reconstruct covers the (n+1)(n+2ng) faces of its
+# direction (the transverse quadrature needs the ghost rows), flux and corner
+# the (n+1)^2 faces/corners, cell_emf and update the n^2 cells, prep the
+# (n+2ng)^2 padded cells.
 #   bytes: every plane the kernel must read once and write once (8 B words),
-#   flops: the arithmetic of the algorithm for one unit (add/mul/div/sqrt
-#          count 1, fma 2), from the per-face operation counts of the
-#          characteristic MP7 reconstruction, 4-point transverse HLLD, the
-#          4x4 tensor-Gauss cell EMF and the 8-tap corner interpolation.
+#   flops: the arithmetic the algorithm needs for one unit (add, mul, div and
+#          sqrt count 1, fma 2): the reference's operation sequence for that
+#          unit, counting shared sub-results (a cell's x-direction point
+#          recovery, used by seven stencil rows) once.
 UNIT_COST = {
-    # 8 state planes + 2 collocated field planes in, 16 interface planes + 1 flag out
-    "reconstruct": {"bytes": 8 * (8 + 2 + 16) + 4, "flops": 6440},
-    # 16 interface planes in, 8 flux planes + 1 rate plane out
-    "flux": {"bytes": 8 * (16 + 8 + 1), "flops": 3520},
-    # 8 state planes + 2 collocated field planes in, 1 EMF plane out
-    "cell_emf": {"bytes": 8 * (8 + 2 + 1), "flops": 4610},
-    # fx.by, fy.bx, cell EMF, 2 flag planes in, corner EMF out
+    # eigensystem 1.3k (0.7k of it the L R = I test), 8 projections 0.8k,
+    # 14 MP7 face values + limiters 0.56k, back-projection 0.2k, checks 0.1k;
+    # 8 state + 2 collocated field planes in, 16 interface planes + flag out
+    "reconstruct": {"bytes": 8 * (8 + 2 + 16) + 4, "flops": 3000},
+    # 16x4 point recoveries 0.9k, 4 HLLD solves 1.2k, admissibility and
+    # wave speeds 0.4k; 16 interface planes in, 8 flux planes + rate out
+    "flux": {"bytes": 8 * (16 + 8 + 1), "flops": 2500},
+    # x-recovery 8x4x7 fma, y-recovery 4x4x8x7 fma, 16 EOS + E_z, Gauss
+    # averages; 6 state + 2 collocated field planes in, cell E_z out
+    "cell_emf": {"bytes": 8 * (6 + 2 + 1), "flops": 2600},
+    # 11 eight-tap edge interpolations; 2 flux, cell E_z, 2 flag planes in, E_z out
     "corner": {"bytes": 8 * (3 + 1) + 2 * 4, "flops": 180},
-    # 8 base + 8 stage + 8+8 flux + 1 EMF planes in, 8 planes out
+    # flux difference, curl, Shu-Osher combine; 8 base + 8 stage + 2x8 flux +
+    # E_z planes in, 8 planes out
     "update": {"bytes": 8 * (8 + 8 + 16 + 1 + 8), "flops": 90},
-    # 8 candidate + 8 base planes in, 2 collocated field planes out
-    "prep": {"bytes": 8 * (8 + 8 + 2), "flops": 260},
+    # two correctly rounded 8-tap collocations, theta and checks; 8 candidate +
+    # 8 base planes in, 2 collocated field planes out
+    "prep": {"bytes": 8 * (8 + 8 + 2), "flops": 1000},
 }
+NG = 4
 
 
 def parse():
@@ -71,6 +80,10 @@ def parse():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the timed region, copy a buffer of this size once (HBM traffic calibration "
                          "for counter runs)")
+    ap.add_argument("--summarize-counters", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="no run: turn two rocprofv3 --pmc outputs (FETCH_SIZE, WRITE_SIZE) of a run made with "
+                         "--copy-probe-mib into HBM bytes per launch per kernel kind, written to "
+                         "profiles/traffic_ot<n>_mp7.json")
     return ap.parse_args()
 
 
@@ -104,8 +117,12 @@ def make_solver(n: int):
 
 
 def units(kind: str, n: int) -> int:
-    if kind in ("reconstruct", "flux"):
-        return (n + 1) * n
+    if kind == "reconstruct":
+        return (n + 1) * (n + 2 * NG)
+    if kind in ("flux", "corner"):
+        return (n + 1) * (n + 1)
+    if kind == "prep":
+        return (n + 2 * NG) * (n + 2 * NG)
     return n * n
 
 
@@ -151,6 +168,54 @@ def measured_traffic(kind: str, n: int):
         return None
 
 
+KERNEL_SYMBOLS = {
+    "prep": "prep_kernel", "reconstruct": "reconstruct_kernel<double, 7>", "flux": "flux_kernel<double, 7>",
+    "cell_emf": "cell_emf_kernel<double, 7>", "corner": "corner_kernel", "update": "update_kernel",
+}
+
+
+def _counter_means(run_dir: str, counter: str):
+    """Mean per-dispatch value of `counter` for every kernel name in a rocprofv3
+    --pmc csv output directory."""
+    import csv
+    import glob
+    sums, counts = {}, {}
+    for path in glob.glob(os.path.join(run_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row["Kernel_Name"]
+                sums[name] = sums.get(name, 0.0) + float(row["Counter_Value"])
+                counts[name] = counts.get(name, 0) + 1
+    return {k: sums[k] / counts[k] for k in sums}
+
+
+def summarize_counters(fetch_dir: str, write_dir: str, n: int, probe_mib: int) -> dict:
+    """HBM bytes per launch of each kernel kind.  FETCH_SIZE and WRITE_SIZE read
+    KiB per dispatch; each is scaled by the factor that makes the copy probe
+    (probe_mib MiB read and written once) read exactly its size."""
+    fetch = _counter_means(fetch_dir, "FETCH_SIZE")
+    write = _counter_means(write_dir, "WRITE_SIZE")
+    probe = [k for k in fetch if "copy" in k.lower() or "elementwise" in k.lower()]
+    if not probe:
+        raise RuntimeError("no copy-probe kernel in the counter output")
+    probe_kib = probe_mib * 1024.0
+    fetch_scale = probe_kib / max(fetch[k] for k in probe)
+    write_scale = probe_kib / max(write.get(k, 0.0) for k in probe)
+    out = {"probe_kernel": max(probe, key=lambda k: fetch[k]), "fetch_scale": fetch_scale,
+           "write_scale": write_scale, "unit": "bytes per launch"}
+    for kind, sym in KERNEL_SYMBOLS.items():
+        names = [k for k in fetch if sym in k]
+        if not names:
+            continue
+        name = names[0]
+        out[kind] = round((fetch[name] * fetch_scale + write.get(name, 0.0) * write_scale) * 1024.0)
+        out[kind + "_fetch_bytes"] = round(fetch[name] * fetch_scale * 1024.0)
+        out[kind + "_write_bytes"] = round(write.get(name, 0.0) * write_scale * 1024.0)
+    return out
+
+
 def cpu_baseline(n: int, budget_s: float):
     """The CPU restatement (OpenMP C++) on Orszag-Tang n^2 MP7 FP64: steps are
     taken until `budget_s` seconds of CPU work have been timed."""
@@ -181,6 +246,13 @@ def cpu_baseline(n: int, budget_s: float):
 
 def main():
     args = parse()
+    if args.summarize_counters:
+        summary = summarize_counters(args.summarize_counters[0], args.summarize_counters[1], args.n,
+                                     args.copy_probe_mib or 1024)
+        path = ROOT / "profiles" / f"traffic_ot{args.n}_mp7.json"
+        path.write_text(json.dumps(summary, indent=1) + "\n")
+        print(json.dumps(summary))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "

@@ -45,6 +45,26 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 __device__ __forceinline__ bool lane_leader() { return (threadIdx.x + blockDim.x * threadIdx.y) % 64 == 0; }
 
+// Reduction over a 256-thread block (4 waves): wave shuffles, then the four
+// wave results through LDS.  Every thread of the block must call it; the
+// result is returned to all of them.  Blocks that reduce cover many cells
+// (kRowsPerThread rows per thread), so one atomic per block keeps the status
+// slots free of contention.
+constexpr int kRowsPerThread = 8;
+template <bool MAX>
+__device__ __forceinline__ double block_reduce(double v, double* lds4) {
+  v = MAX ? wave_max(v) : wave_min(v);
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  if (tid % 64 == 0) lds4[tid / 64] = v;
+  __syncthreads();
+  double r = lds4[0];
+#pragma unroll
+  for (int w = 1; w < 4; ++w) r = MAX ? fmax(r, lds4[w]) : fmin(r, lds4[w]);
+  __syncthreads();
+  return r;
+}
+__device__ __forceinline__ bool block_leader() { return threadIdx.x == 0 && threadIdx.y == 0; }
+
 // Internal energy q = E - K - M of a convex combination (the admissible set
 // rho > 0, q > 0 is convex).
 __device__ __forceinline__ double convex_q(const St<double>& a, const St<double>& b, double th) {
@@ -101,82 +121,97 @@ __device__ __forceinline__ double ulp_of(double v) {
 // live-state admissibility under both collocations, (c) the relative div(B)
 // residual/scale maxima.  Results go to `status` through order-independent
 // atomics.
+__device__ __forceinline__ double ok_or_zero(double v) { return fin(v) ? v : 0.0; }
+
+// Collocation of the staggered field (every padded cell), the admissible
+// fraction of a stage candidate against its base (status.theta[slot]; with
+// theta_both the other collocation into theta[3]) and the live-state
+// preflights (admissibility under both collocations, the relative div B
+// residual).  Each thread covers kRowsPerThread rows.
 __global__ __launch_bounds__(256) void prep_kernel(GridD g, StateView u, double* __restrict__ bxc,
                                                    double* __restrict__ byc, StateView base,
                                                    const double* __restrict__ base_bxc,
                                                    const double* __restrict__ base_byc, int theta_slot,
                                                    int theta_co, int theta_both, int live_checks,
                                                    int solver_owned, StepStatus* __restrict__ status) {
+  __shared__ double lds[4];
   const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) - g.ng;
-  const int j = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y) - g.ng;
-  const bool in_pad = i < g.nx + g.ng && j < g.ny + g.ng;
-  double bxv = 0.0, byv = 0.0;
-  if (in_pad) {
-    bxv = face_to_cell(g, u.c[5], 0, i, j);
-    byv = face_to_cell(g, u.c[6], 1, j, i);
-    const std::size_t k = idx(g, i, j);
-    bxc[k] = bxv;
-    byc[k] = byv;
-  }
-  const bool interior = in_pad && i >= 0 && j >= 0 && i < g.nx && j < g.ny;
   double th = 1.0, th_other = 1.0, live0 = 1.0, live1 = 1.0, res = 0.0, scl = 0.0;
-  if (interior) {
-    const std::size_t k = idx(g, i, j);
-    const St<double> cu = cell_state(g, u, k, bxv, byv);
-    if (theta_slot >= 0) {
-      double t0 = 1.0, t1 = 1.0;
-      if (theta_co == 0 || theta_both) {
-        const St<double> cb = cell_state(g, base, k, base_bxc[k], base_byc[k]);
-        t0 = admissible_theta(cb, cu);
+  for (int r = 0; r < kRowsPerThread; ++r) {
+    const int j = static_cast<int>((blockIdx.y * kRowsPerThread + r) * blockDim.y + threadIdx.y) - g.ng;
+    const bool in_pad = i < g.nx + g.ng && j < g.ny + g.ng;
+    double bxv = 0.0, byv = 0.0;
+    if (in_pad) {
+      bxv = face_to_cell(g, u.c[5], 0, i, j);
+      byv = face_to_cell(g, u.c[6], 1, j, i);
+      const std::size_t k = idx(g, i, j);
+      bxc[k] = bxv;
+      byc[k] = byv;
+    }
+    const bool interior = in_pad && i >= 0 && j >= 0 && i < g.nx && j < g.ny;
+    if (interior) {
+      const std::size_t k = idx(g, i, j);
+      const St<double> cu = cell_state(g, u, k, bxv, byv);
+      if (theta_slot >= 0) {
+        double t0 = 1.0, t1 = 1.0;
+        if (theta_co == 0 || theta_both) {
+          const St<double> cb = cell_state(g, base, k, base_bxc[k], base_byc[k]);
+          t0 = admissible_theta(cb, cu);
+        }
+        if (theta_co == 1 || theta_both) {
+          const St<double> cb1 = cell_state(g, base, k, face_to_cell_lo(g, base.c[5], 0, i, j),
+                                            face_to_cell_lo(g, base.c[6], 1, j, i));
+          const St<double> cu1 = cell_state(g, u, k, face_to_cell_lo(g, u.c[5], 0, i, j),
+                                            face_to_cell_lo(g, u.c[6], 1, j, i));
+          t1 = admissible_theta(cb1, cu1);
+        }
+        th = fmin(th, ok_or_zero(theta_co == 0 ? t0 : t1));
+        th_other = fmin(th_other, ok_or_zero(theta_co == 0 ? t1 : t0));
       }
-      if (theta_co == 1 || theta_both) {
-        const St<double> cb1 = cell_state(g, base, k, face_to_cell_lo(g, base.c[5], 0, i, j),
-                                          face_to_cell_lo(g, base.c[6], 1, j, i));
+      if (live_checks) {
+        live0 = fmin(live0, ok_or_zero(admissible_theta(cu, cu)));
         const St<double> cu1 = cell_state(g, u, k, face_to_cell_lo(g, u.c[5], 0, i, j),
                                           face_to_cell_lo(g, u.c[6], 1, j, i));
-        t1 = admissible_theta(cb1, cu1);
-      }
-      th = theta_co == 0 ? t0 : t1;
-      th_other = theta_co == 0 ? t1 : t0;
-    }
-    if (live_checks) {
-      live0 = admissible_theta(cu, cu);
-      const St<double> cu1 = cell_state(g, u, k, face_to_cell_lo(g, u.c[5], 0, i, j),
-                                        face_to_cell_lo(g, u.c[6], 1, j, i));
-      live1 = admissible_theta(cu1, cu1);
-      const double bxl = u.c[5][k], bxh = u.c[5][idx(g, i + 1, j)];
-      const double byl = u.c[6][k], byh = u.c[6][idx(g, i, j + 1)];
-      const double radial = (bxh - bxl) / g.dx;
-      const double axial = (byh - byl) / g.dy;
-      if (!(fin(radial) && fin(axial))) {
-        res = Lim<double>::inf();
-        scl = Lim<double>::inf();
-      } else {
-        const double r = radial + axial;
-        const double s = __builtin_fabs(radial) + __builtin_fabs(axial);
-        const double ro = (ulp_of(bxh) / g.dx + ulp_of(bxl) / g.dx) + (ulp_of(byh) / g.dy + ulp_of(byl) / g.dy);
-        bool explained;
-        if (solver_owned) {
-          explained = r == 0.0 || (ro != 0.0 && __builtin_fabs(r) <= ldexp(ro, 10));
-        } else if (radial == 0.0 || axial == 0.0 || signbit(radial) == signbit(axial)) {
-          explained = r == 0.0;
+        live1 = fmin(live1, ok_or_zero(admissible_theta(cu1, cu1)));
+        const double bxl = u.c[5][k], bxh = u.c[5][idx(g, i + 1, j)];
+        const double byl = u.c[6][k], byh = u.c[6][idx(g, i, j + 1)];
+        const double radial = (bxh - bxl) / g.dx;
+        const double axial = (byh - byl) / g.dy;
+        if (!(fin(radial) && fin(axial))) {
+          res = Lim<double>::inf();
+          scl = Lim<double>::inf();
         } else {
-          explained = r == 0.0 || (__builtin_fabs(r) <= 0.5 * s && ro != 0.0 && __builtin_fabs(r) <= ldexp(ro, 10));
+          const double r = radial + axial;
+          const double s = __builtin_fabs(radial) + __builtin_fabs(axial);
+          const double ro = (ulp_of(bxh) / g.dx + ulp_of(bxl) / g.dx) + (ulp_of(byh) / g.dy + ulp_of(byl) / g.dy);
+          bool explained;
+          if (solver_owned) {
+            explained = r == 0.0 || (ro != 0.0 && __builtin_fabs(r) <= ldexp(ro, 10));
+          } else if (radial == 0.0 || axial == 0.0 || signbit(radial) == signbit(axial)) {
+            explained = r == 0.0;
+          } else {
+            explained = r == 0.0 || (__builtin_fabs(r) <= 0.5 * s && ro != 0.0 && __builtin_fabs(r) <= ldexp(ro, 10));
+          }
+          res = fmax(res, explained ? 0.0 : __builtin_fabs(r));
+          scl = fmax(scl, s);
         }
-        res = explained ? 0.0 : __builtin_fabs(r);
-        scl = s;
       }
     }
   }
-  th = wave_min(fin(th) ? th : 0.0);
-  th_other = wave_min(fin(th_other) ? th_other : 0.0);
-  live0 = wave_min(fin(live0) ? live0 : 0.0);
-  live1 = wave_min(fin(live1) ? live1 : 0.0);
-  res = wave_max(res);
-  scl = wave_max(scl);
-  if (lane_leader()) {
-    if (theta_slot >= 0) atomic_min_nonneg(&status->theta[theta_slot], th);
-    if (theta_slot >= 0 && theta_both) atomic_min_nonneg(&status->theta[3], th_other);
+  const bool reduce_theta = theta_slot >= 0;
+  if (reduce_theta) {
+    th = block_reduce<false>(th, lds);
+    if (theta_both) th_other = block_reduce<false>(th_other, lds);
+  }
+  if (live_checks) {
+    live0 = block_reduce<false>(live0, lds);
+    live1 = block_reduce<false>(live1, lds);
+    res = block_reduce<true>(res, lds);
+    scl = block_reduce<true>(scl, lds);
+  }
+  if (block_leader()) {
+    if (reduce_theta) atomic_min_nonneg(&status->theta[theta_slot], th);
+    if (reduce_theta && theta_both) atomic_min_nonneg(&status->theta[3], th_other);
     if (live_checks) {
       atomic_min_nonneg(&status->live_theta0, live0);
       atomic_min_nonneg(&status->live_theta1, live1);
@@ -343,22 +378,30 @@ __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_onl
     if (!ok) {
       f = riemann(baseL, baseR, dir, hll_only, gamma);
     } else {
-      St<T> fq[NQ];
+      // Gauss average of the point fluxes, accumulated point by point in the
+      // same summation order, with the exact constant-flux shortcut (all
+      // point fluxes equal -> that flux) tracked as a running flag.
+      T acc[8], first[8];
+      bool same[8];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) fq[q] = riemann(pl[q], pr[q], dir, hll_only, gamma);
+      for (int q = 0; q < NQ; ++q) {
+        const St<T> fq = riemann(pl[q], pr[q], dir, hll_only, gamma);
+        const T w = T(ORDER == 5 ? c_rows.mp5_gauss[q] : c_rows.mp7_gauss[q]);
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        bool constant = true;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) constant = constant && comp(fq[q], c) == comp(fq[NQ / 2], c);
-        T v = comp(fq[NQ / 2], c);
-        if (!constant) {
-          v = T(0);
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) v += T(ORDER == 5 ? c_rows.mp5_gauss[q] : c_rows.mp7_gauss[q]) * comp(fq[q], c);
+        for (int c = 0; c < 8; ++c) {
+          const T v = comp(fq, c);
+          if (q == 0) {
+            first[c] = v;
+            same[c] = v == v;
+            acc[c] = w * v;
+          } else {
+            same[c] = same[c] && v == first[c];
+            acc[c] += w * v;
+          }
         }
-        set_comp(f, c, v);
       }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) set_comp(f, c, same[c] ? first[c] : acc[c]);
     }
   }
 #pragma unroll
@@ -371,17 +414,20 @@ __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_onl
 __global__ __launch_bounds__(256) void cfl_reduce_kernel(GridD g, const double* __restrict__ rate_x,
                                                          const double* __restrict__ rate_y,
                                                          unsigned long long* __restrict__ max_rate) {
+  __shared__ double lds[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
   double local = 0.0;
-  if (i < g.nx && j < g.ny) {
-    const double rx = rate_x[idx(g, i, j)] + rate_x[idx(g, i + 1, j)];
-    const double ry = rate_y[idx(g, i, j)] + rate_y[idx(g, i, j + 1)];
-    local = rx + ry;
-    local = fin(local) ? fmax(0.0, local) : Lim<double>::inf();
+  for (int r = 0; r < kRowsPerThread; ++r) {
+    const int j = (blockIdx.y * kRowsPerThread + r) * blockDim.y + threadIdx.y;
+    if (i < g.nx && j < g.ny) {
+      const double rx = rate_x[idx(g, i, j)] + rate_x[idx(g, i + 1, j)];
+      const double ry = rate_y[idx(g, i, j)] + rate_y[idx(g, i, j + 1)];
+      const double v = rx + ry;
+      local = fmax(local, fin(v) ? fmax(0.0, v) : Lim<double>::inf());
+    }
   }
-  local = wave_max(local);
-  if (lane_leader()) atomic_max_nonneg(max_rate, local);
+  local = block_reduce<true>(local, lds);
+  if (block_leader()) atomic_max_nonneg(max_rate, local);
 }
 
 // ---------------------------------------------------------------------------
@@ -401,7 +447,10 @@ __global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high,
   constexpr int TW = 2 * H + 1;
   constexpr int TX = kEmfBx + 2 * H;
   constexpr int TY = kEmfBy + 2 * H;
+  // The block's cell averages with an H halo, and the x-direction point values
+  // of one x node for every tile row of the block's columns.
   __shared__ T tile[8][TY][TX];
+  __shared__ T xrec[8][TY][kEmfBx];
   const int i0 = blockIdx.x * kEmfBx, j0 = blockIdx.y * kEmfBy;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   for (int t = tid; t < TX * TY; t += blockDim.x * blockDim.y) {
@@ -420,94 +469,49 @@ __global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high,
   }
   __syncthreads();
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-  if (i > g.nx || j > g.ny) return;
-  if ((i == g.nx && !phys_x_high) || (j == g.ny && !phys_y_high)) return;
-  const int a0 = threadIdx.x + H, b0 = threadIdx.y + H;
+  const bool active = i <= g.nx && j <= g.ny && !(i == g.nx && !phys_x_high) && !(j == g.ny && !phys_y_high);
+  // Stencil bounds inside the padded domain (always true for the computed
+  // cells when ng >= H + 1).
+  const bool in_range = i - H >= -g.ng && i + H <= g.nx + g.ng - 1 && j - H >= -g.ng && j + H <= g.ny + g.ng - 1;
   const T gm1 = gamma - T(1);
-  const T fallback = physical_ez(tile[0][b0][a0], tile[1][b0][a0], tile[2][b0][a0], tile[5][b0][a0], tile[6][b0][a0]);
-  T ez = fallback;
-  if (ORDER == 5 || ORDER == 7) {
-    // Stencil bounds inside the padded domain (always true for interior and
-    // high-closure cells when ng >= H + 1).
-    const bool in_range = i - H >= -g.ng && i + H <= g.nx + g.ng - 1 && j - H >= -g.ng && j + H <= g.ny + g.ng - 1;
-    if (in_range) {
-      T point[NQ][NQ];
-      bool ok = true;
-      for (int qx = 0; qx < NQ && ok; ++qx) {
-        T wx[TW];
+  T point[NQ][NQ];
+  bool ok = in_range;
+  for (int qx = 0; qx < NQ; ++qx) {
+    const double* wx = ORDER == 5 ? c_rows.mp5_point[qx] : c_rows.mp7_point[qx];
+    // Pass 1: x-direction point value of every (tile row, block column), each once.
+    for (int t = tid; t < TY * kEmfBx; t += blockDim.x * blockDim.y) {
+      const int r = t / kEmfBx, col = t % kEmfBx;
 #pragma unroll
-        for (int t = 0; t < TW; ++t) wx[t] = T(ORDER == 5 ? c_rows.mp5_point[qx][t] : c_rows.mp7_point[qx][t]);
-        // rows[c][r]: x-recovery of component c on stencil row r.
-        T rows[8][TW];
+      for (int c = 0; c < 8; ++c) xrec[c][r][col] = point_from_averages<T, TW>(&tile[c][r][col], wx);
+    }
+    __syncthreads();
+    // Pass 2: y-direction recovery of this cell's NQ nodes at x node qx.
+    if (active && ok) {
+      T s[8][TW];
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
+      for (int c = 0; c < 8; ++c)
 #pragma unroll
-          for (int r = 0; r < TW; ++r) {
-            const T* line = &tile[c][b0 - H + r][a0 - H];
-            bool constant = true;
+        for (int r = 0; r < TW; ++r) s[c][r] = xrec[c][threadIdx.y + r][threadIdx.x];
 #pragma unroll
-            for (int t = 0; t < TW; ++t) constant = constant && line[t] == line[H];
-            T v = line[H];
-            if (!constant) {
-              v = T(0);
+      for (int qy = 0; qy < NQ; ++qy) {
+        const double* wy = ORDER == 5 ? c_rows.mp5_point[qy] : c_rows.mp7_point[qy];
+        St<T> st;
 #pragma unroll
-              for (int t = 0; t < TW; ++t) v += wx[t] * line[t];
-            }
-            rows[c][r] = v;
-          }
-#pragma unroll
-        for (int qy = 0; qy < NQ; ++qy) {
-          St<T> st;
-#pragma unroll
-          for (int c = 0; c < 8; ++c) {
-            bool constant = true;
-#pragma unroll
-            for (int r = 0; r < TW; ++r) constant = constant && rows[c][r] == rows[c][H];
-            T v = rows[c][H];
-            if (!constant) {
-              v = T(0);
-#pragma unroll
-              for (int r = 0; r < TW; ++r)
-                v += T(ORDER == 5 ? c_rows.mp5_point[qy][r] : c_rows.mp7_point[qy][r]) * rows[c][r];
-            }
-            set_comp(st, c, v);
-          }
-          const T p = pressure(st, gm1);
-          const bool adm = fin(st.rho) && st.rho > T(0) && fin(st.mx) && fin(st.my) && fin(st.mz) && fin(st.en) &&
-                           fin(st.bx) && fin(st.by) && fin(st.bz) && fin(p) && p > T(0);
-          if (!adm) {
-            ok = false;
-            break;
-          }
-          point[qy][qx] = physical_ez(st.rho, st.mx, st.my, st.bx, st.by);
-        }
-      }
-      if (ok) {
-        T row[NQ];
-#pragma unroll
-        for (int qy = 0; qy < NQ; ++qy) {
-          bool constant = true;
-#pragma unroll
-          for (int qx = 0; qx < NQ; ++qx) constant = constant && point[qy][qx] == point[qy][NQ / 2];
-          T v = point[qy][NQ / 2];
-          if (!constant) {
-            v = T(0);
-#pragma unroll
-            for (int qx = 0; qx < NQ; ++qx) v += T(ORDER == 5 ? c_rows.mp5_gauss[qx] : c_rows.mp7_gauss[qx]) * point[qy][qx];
-          }
-          row[qy] = v;
-        }
-        bool constant = true;
-#pragma unroll
-        for (int qy = 0; qy < NQ; ++qy) constant = constant && row[qy] == row[NQ / 2];
-        ez = row[NQ / 2];
-        if (!constant) {
-          ez = T(0);
-#pragma unroll
-          for (int qy = 0; qy < NQ; ++qy) ez += T(ORDER == 5 ? c_rows.mp5_gauss[qy] : c_rows.mp7_gauss[qy]) * row[qy];
-        }
+        for (int c = 0; c < 8; ++c) set_comp(st, c, point_from_averages<T, TW>(s[c], wy));
+        ok = ok && point_ez(st, gm1, point[qy][qx]);
       }
     }
+    __syncthreads();
+  }
+  if (!active) return;
+  const int a0 = threadIdx.x + H, b0 = threadIdx.y + H;
+  T ez = physical_ez(tile[0][b0][a0], tile[1][b0][a0], tile[2][b0][a0], tile[5][b0][a0], tile[6][b0][a0]);
+  if (ok) {
+    const double* wg = ORDER == 5 ? c_rows.mp5_gauss : c_rows.mp7_gauss;
+    T row[NQ];
+#pragma unroll
+    for (int qy = 0; qy < NQ; ++qy) row[qy] = point_from_averages<T, NQ>(point[qy], wg);
+    ez = point_from_averages<T, NQ>(row, wg);
   }
   cell_ez[idx(g, i, j)] = static_cast<double>(ez);
 }
@@ -893,34 +897,39 @@ __global__ __launch_bounds__(256) void rk_combine_kernel(GridD g, double a, doub
 }
 
 __global__ __launch_bounds__(256) void divb_linf_kernel(GridD g, StateView u, unsigned long long* out) {
+  __shared__ double lds[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
   double local = 0.0;
-  if (i < g.nx && j < g.ny) {
-    const double d = (u.c[5][idx(g, i + 1, j)] - u.c[5][idx(g, i, j)]) / g.dx +
-                     (u.c[6][idx(g, i, j + 1)] - u.c[6][idx(g, i, j)]) / g.dy;
-    local = fin(d) ? __builtin_fabs(d) : Lim<double>::inf();
+  for (int r = 0; r < kRowsPerThread; ++r) {
+    const int j = (blockIdx.y * kRowsPerThread + r) * blockDim.y + threadIdx.y;
+    if (i < g.nx && j < g.ny) {
+      const double d = (u.c[5][idx(g, i + 1, j)] - u.c[5][idx(g, i, j)]) / g.dx +
+                       (u.c[6][idx(g, i, j + 1)] - u.c[6][idx(g, i, j)]) / g.dy;
+      local = fmax(local, fin(d) ? __builtin_fabs(d) : Lim<double>::inf());
+    }
   }
-  local = wave_max(local);
-  if (lane_leader()) atomic_max_nonneg(out, local);
+  local = block_reduce<true>(local, lds);
+  if (block_leader()) atomic_max_nonneg(out, local);
 }
 
 __global__ __launch_bounds__(256) void theta_kernel(GridD g, StateView base, StateView cand, int co,
                                                     unsigned long long* out) {
+  __shared__ double lds[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
   double th = 1.0;
-  if (i < g.nx && j < g.ny) {
-    const std::size_t k = idx(g, i, j);
-    const double bxa = co == 1 ? face_to_cell_lo(g, base.c[5], 0, i, j) : face_to_cell(g, base.c[5], 0, i, j);
-    const double bya = co == 1 ? face_to_cell_lo(g, base.c[6], 1, j, i) : face_to_cell(g, base.c[6], 1, j, i);
-    const double bxb = co == 1 ? face_to_cell_lo(g, cand.c[5], 0, i, j) : face_to_cell(g, cand.c[5], 0, i, j);
-    const double byb = co == 1 ? face_to_cell_lo(g, cand.c[6], 1, j, i) : face_to_cell(g, cand.c[6], 1, j, i);
-    th = admissible_theta(cell_state(g, base, k, bxa, bya), cell_state(g, cand, k, bxb, byb));
-    th = fin(th) ? th : 0.0;
+  for (int r = 0; r < kRowsPerThread; ++r) {
+    const int j = (blockIdx.y * kRowsPerThread + r) * blockDim.y + threadIdx.y;
+    if (i < g.nx && j < g.ny) {
+      const std::size_t k = idx(g, i, j);
+      const double bxa = co == 1 ? face_to_cell_lo(g, base.c[5], 0, i, j) : face_to_cell(g, base.c[5], 0, i, j);
+      const double bya = co == 1 ? face_to_cell_lo(g, base.c[6], 1, j, i) : face_to_cell(g, base.c[6], 1, j, i);
+      const double bxb = co == 1 ? face_to_cell_lo(g, cand.c[5], 0, i, j) : face_to_cell(g, cand.c[5], 0, i, j);
+      const double byb = co == 1 ? face_to_cell_lo(g, cand.c[6], 1, j, i) : face_to_cell(g, cand.c[6], 1, j, i);
+      th = fmin(th, ok_or_zero(admissible_theta(cell_state(g, base, k, bxa, bya), cell_state(g, cand, k, bxb, byb))));
+    }
   }
-  th = wave_min(th);
-  if (lane_leader()) atomic_min_nonneg(out, th);
+  th = block_reduce<false>(th, lds);
+  if (block_leader()) atomic_min_nonneg(out, th);
 }
 
 // Per-block compensated (two-sum) partial sums of mass and energy; the host
@@ -1050,7 +1059,8 @@ void launch_prep(const GridD& g, const StateView& u, double* bxc, double* byc, c
                  const double* base_bxc, const double* base_byc, int theta_slot, int theta_co, int theta_both,
                  int live_checks, int solver_owned, StepStatus* status, hipStream_t stream) {
   const dim3 b(64, 4);
-  hipLaunchKernelGGL(prep_kernel, blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, b), b, 0, stream, g, u, bxc, byc, base,
+  hipLaunchKernelGGL(prep_kernel, blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, dim3(64, 4 * kRowsPerThread)), b, 0,
+                     stream, g, u, bxc, byc, base,
                      base_bxc, base_byc, theta_slot, theta_co, theta_both, live_checks, solver_owned, status);
 }
 
@@ -1109,7 +1119,8 @@ void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4&
 void launch_cfl_reduce(const GridD& g, const double* rate_x, const double* rate_y, unsigned long long* max_rate,
                        hipStream_t stream) {
   const dim3 b(64, 4);
-  hipLaunchKernelGGL(cfl_reduce_kernel, blocks2d(g.nx, g.ny, b), b, 0, stream, g, rate_x, rate_y, max_rate);
+  hipLaunchKernelGGL(cfl_reduce_kernel, blocks2d(g.nx, g.ny, dim3(64, 4 * kRowsPerThread)), b, 0, stream, g, rate_x,
+                     rate_y, max_rate);
 }
 
 void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamma, const StateView& u,
@@ -1175,13 +1186,14 @@ void launch_seed(const GridD& g, const IcSpecD& s, const StateOut& out, double* 
 
 void launch_divb_linf(const GridD& g, const StateView& u, unsigned long long* out, hipStream_t stream) {
   const dim3 b(64, 4);
-  hipLaunchKernelGGL(divb_linf_kernel, blocks2d(g.nx, g.ny, b), b, 0, stream, g, u, out);
+  hipLaunchKernelGGL(divb_linf_kernel, blocks2d(g.nx, g.ny, dim3(64, 4 * kRowsPerThread)), b, 0, stream, g, u, out);
 }
 
 void launch_theta(const GridD& g, const StateView& base, const StateView& cand, int co, unsigned long long* out,
                   hipStream_t stream) {
   const dim3 b(64, 4);
-  hipLaunchKernelGGL(theta_kernel, blocks2d(g.nx, g.ny, b), b, 0, stream, g, base, cand, co, out);
+  hipLaunchKernelGGL(theta_kernel, blocks2d(g.nx, g.ny, dim3(64, 4 * kRowsPerThread)), b, 0, stream, g, base, cand, co,
+                     out);
 }
 
 void launch_cell_sums(const GridD& g, const StateView& u, double* partials, int blocks, hipStream_t stream) {

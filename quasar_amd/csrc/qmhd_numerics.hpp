@@ -294,45 +294,6 @@ __host__ __device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<
   const T Nf = T(0.5) * inv_a2;
   const T rho_a2 = gamma * p;
 
-  // Primitive right eigenvectors Rp[wave][var], var = (rho,vn,vt1,vt2,p,bt1,bt2),
-  // and left eigenvectors Lp[wave][var]; unset entries are zero.
-  T Rp[7][7];
-  T Lp[7][7];
-#pragma unroll
-  for (int k = 0; k < 7; ++k)
-#pragma unroll
-    for (int v = 0; v < 7; ++v) {
-      Rp[k][v] = T(0);
-      Lp[k][v] = T(0);
-    }
-  Rp[0][0] = rho * alpha_f; Rp[0][1] = -alpha_f * cf; Rp[0][2] = qs * beta1; Rp[0][3] = qs * beta2;
-  Rp[0][4] = alpha_f * rho_a2; Rp[0][5] = As * beta1; Rp[0][6] = As * beta2;
-  Rp[6][0] = rho * alpha_f; Rp[6][1] = alpha_f * cf; Rp[6][2] = -qs * beta1; Rp[6][3] = -qs * beta2;
-  Rp[6][4] = alpha_f * rho_a2; Rp[6][5] = As * beta1; Rp[6][6] = As * beta2;
-  Rp[1][2] = -beta2 * rs2; Rp[1][3] = beta1 * rs2;
-  Rp[1][5] = -sbn * beta2 * rs2 * sqrt_rho; Rp[1][6] = sbn * beta1 * rs2 * sqrt_rho;
-  Rp[5][2] = -beta2 * rs2; Rp[5][3] = beta1 * rs2;
-  Rp[5][5] = sbn * beta2 * rs2 * sqrt_rho; Rp[5][6] = -sbn * beta1 * rs2 * sqrt_rho;
-  Rp[2][0] = rho * alpha_s; Rp[2][1] = -alpha_s * cs; Rp[2][2] = -qf * beta1; Rp[2][3] = -qf * beta2;
-  Rp[2][4] = alpha_s * rho_a2; Rp[2][5] = -Af * beta1; Rp[2][6] = -Af * beta2;
-  Rp[4][0] = rho * alpha_s; Rp[4][1] = alpha_s * cs; Rp[4][2] = qf * beta1; Rp[4][3] = qf * beta2;
-  Rp[4][4] = alpha_s * rho_a2; Rp[4][5] = -Af * beta1; Rp[4][6] = -Af * beta2;
-  Rp[3][0] = T(1);
-
-  Lp[0][1] = -Nf * alpha_f * cf; Lp[0][2] = Nf * qs * beta1; Lp[0][3] = Nf * qs * beta2;
-  Lp[0][4] = Nf * alpha_f / rho; Lp[0][5] = Nf * As * beta1 / rho; Lp[0][6] = Nf * As * beta2 / rho;
-  Lp[6][1] = Nf * alpha_f * cf; Lp[6][2] = -Nf * qs * beta1; Lp[6][3] = -Nf * qs * beta2;
-  Lp[6][4] = Nf * alpha_f / rho; Lp[6][5] = Nf * As * beta1 / rho; Lp[6][6] = Nf * As * beta2 / rho;
-  Lp[1][2] = -beta2 * rs2; Lp[1][3] = beta1 * rs2;
-  Lp[1][5] = -sbn * beta2 * rs2 / sqrt_rho; Lp[1][6] = sbn * beta1 * rs2 / sqrt_rho;
-  Lp[5][2] = -beta2 * rs2; Lp[5][3] = beta1 * rs2;
-  Lp[5][5] = sbn * beta2 * rs2 / sqrt_rho; Lp[5][6] = -sbn * beta1 * rs2 / sqrt_rho;
-  Lp[2][1] = -Nf * alpha_s * cs; Lp[2][2] = -Nf * qf * beta1; Lp[2][3] = -Nf * qf * beta2;
-  Lp[2][4] = Nf * alpha_s / rho; Lp[2][5] = -Nf * Af * beta1 / rho; Lp[2][6] = -Nf * Af * beta2 / rho;
-  Lp[4][1] = Nf * alpha_s * cs; Lp[4][2] = Nf * qf * beta1; Lp[4][3] = Nf * qf * beta2;
-  Lp[4][4] = Nf * alpha_s / rho; Lp[4][5] = -Nf * Af * beta1 / rho; Lp[4][6] = -Nf * Af * beta2 / rho;
-  Lp[3][0] = T(1); Lp[3][4] = -inv_a2;
-
   // M = dU/dW and its inverse (mhd_eigensystem.hpp:333-379): only their
   // non-zero entries enter the products below, in the reference's j order.
   const T gm1 = gamma - T(1);
@@ -342,9 +303,48 @@ __host__ __device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<
   const T i10 = -vn * inv_rho, i20 = -vt1 * inv_rho, i30 = -vt2 * inv_rho;
   const T i40 = gm1 * half_v2, i41 = -gm1 * vn, i42 = -gm1 * vt1, i43 = -gm1 * vt2;
   const T i45 = -gm1 * bt1, i46 = -gm1 * bt2;
+  // Wave by wave: the primitive right/left eigenvector rows Rp[k][var] and
+  // Lp[k][var], var = (rho, vn, vt1, vt2, p, bt1, bt2), unset entries zero,
+  // are formed where they are transformed, so only the conserved L and R
+  // stay live.
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
-    const T* r = Rp[k];
+    T r[7] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+    T l[7] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (k == 0) {
+      r[0] = rho * alpha_f; r[1] = -alpha_f * cf; r[2] = qs * beta1; r[3] = qs * beta2;
+      r[4] = alpha_f * rho_a2; r[5] = As * beta1; r[6] = As * beta2;
+      l[1] = -Nf * alpha_f * cf; l[2] = Nf * qs * beta1; l[3] = Nf * qs * beta2;
+      l[4] = Nf * alpha_f / rho; l[5] = Nf * As * beta1 / rho; l[6] = Nf * As * beta2 / rho;
+    } else if (k == 6) {
+      r[0] = rho * alpha_f; r[1] = alpha_f * cf; r[2] = -qs * beta1; r[3] = -qs * beta2;
+      r[4] = alpha_f * rho_a2; r[5] = As * beta1; r[6] = As * beta2;
+      l[1] = Nf * alpha_f * cf; l[2] = -Nf * qs * beta1; l[3] = -Nf * qs * beta2;
+      l[4] = Nf * alpha_f / rho; l[5] = Nf * As * beta1 / rho; l[6] = Nf * As * beta2 / rho;
+    } else if (k == 1) {
+      r[2] = -beta2 * rs2; r[3] = beta1 * rs2;
+      r[5] = -sbn * beta2 * rs2 * sqrt_rho; r[6] = sbn * beta1 * rs2 * sqrt_rho;
+      l[2] = -beta2 * rs2; l[3] = beta1 * rs2;
+      l[5] = -sbn * beta2 * rs2 / sqrt_rho; l[6] = sbn * beta1 * rs2 / sqrt_rho;
+    } else if (k == 5) {
+      r[2] = -beta2 * rs2; r[3] = beta1 * rs2;
+      r[5] = sbn * beta2 * rs2 * sqrt_rho; r[6] = -sbn * beta1 * rs2 * sqrt_rho;
+      l[2] = -beta2 * rs2; l[3] = beta1 * rs2;
+      l[5] = sbn * beta2 * rs2 / sqrt_rho; l[6] = -sbn * beta1 * rs2 / sqrt_rho;
+    } else if (k == 2) {
+      r[0] = rho * alpha_s; r[1] = -alpha_s * cs; r[2] = -qf * beta1; r[3] = -qf * beta2;
+      r[4] = alpha_s * rho_a2; r[5] = -Af * beta1; r[6] = -Af * beta2;
+      l[1] = -Nf * alpha_s * cs; l[2] = -Nf * qf * beta1; l[3] = -Nf * qf * beta2;
+      l[4] = Nf * alpha_s / rho; l[5] = -Nf * Af * beta1 / rho; l[6] = -Nf * Af * beta2 / rho;
+    } else if (k == 4) {
+      r[0] = rho * alpha_s; r[1] = alpha_s * cs; r[2] = qf * beta1; r[3] = qf * beta2;
+      r[4] = alpha_s * rho_a2; r[5] = -Af * beta1; r[6] = -Af * beta2;
+      l[1] = Nf * alpha_s * cs; l[2] = Nf * qf * beta1; l[3] = Nf * qf * beta2;
+      l[4] = Nf * alpha_s / rho; l[5] = -Nf * Af * beta1 / rho; l[6] = -Nf * Af * beta2 / rho;
+    } else {
+      r[0] = T(1);
+      l[0] = T(1); l[4] = -inv_a2;
+    }
     e.R[0][k] = r[0];
     e.R[1][k] = vn * r[0] + rho * r[1];
     e.R[2][k] = vt1 * r[0] + rho * r[2];
@@ -352,7 +352,6 @@ __host__ __device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<
     e.R[4][k] = (((((half_v2 * r[0] + m41 * r[1]) + m42 * r[2]) + m43 * r[3]) + m44 * r[4]) + bt1 * r[5]) + bt2 * r[6];
     e.R[5][k] = r[5];
     e.R[6][k] = r[6];
-    const T* l = Lp[k];
     e.L[k][0] = (((l[0] + l[1] * i10) + l[2] * i20) + l[3] * i30) + l[4] * i40;
     e.L[k][1] = l[1] * inv_rho + l[4] * i41;
     e.L[k][2] = l[2] * inv_rho + l[4] * i42;
@@ -447,7 +446,7 @@ __host__ __device__ __forceinline__ bool resolved(T value, T scale) {
 }
 
 template <class T>
-__host__ __device__ __noinline__ St<T> hlld_flux(const St<T>& Lin, const St<T>& Rin, T gamma) {
+__host__ __device__ __forceinline__ St<T> hlld_flux(const St<T>& Lin, const St<T>& Rin, T gamma) {
   const T gm1 = gamma - T(1);
   if (Lin.rho == Rin.rho && Lin.mx == Rin.mx && Lin.my == Rin.my && Lin.mz == Rin.mz &&
       Lin.en == Rin.en && Lin.bx == Rin.bx && Lin.by == Rin.by && Lin.bz == Rin.bz)

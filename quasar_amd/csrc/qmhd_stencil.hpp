@@ -356,25 +356,32 @@ __host__ __device__ __forceinline__ void mp_char_face(const GridD& g, int dir, i
   constexpr int UW = 2 * H + 2;
   constexpr int W = 2 * H + 1;
   const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
-  St<T> cells[UW];
-#pragma unroll
-  for (int q = 0; q < UW; ++q) cells[q] = rot(load_cell<T>(g, u, bxc, byc, i + (q - 1 - H) * di, j + (q - 1 - H) * dj), dir);
+  auto cell = [&](int q) {
+    return rot(load_cell<T>(g, u, bxc, byc, i + (q - 1 - H) * di, j + (q - 1 - H) * dj), dir);
+  };
   St<T> ref;
+  {
+    const St<T> cl = cell(H), cr = cell(H + 1);
 #pragma unroll
-  for (int c = 0; c < 8; ++c) set_comp(ref, c, T(0.5) * comp(cells[H], c) + T(0.5) * comp(cells[H + 1], c));
+    for (int c = 0; c < 8; ++c) set_comp(ref, c, T(0.5) * comp(cl, c) + T(0.5) * comp(cr, c));
+  }
   Eig<T> e;
   eig_build(ref, gamma, e);
-  // Deltas in the 7-vector (rho, mn, mt1, mt2, E, bt1, bt2).
-  T d[UW][7];
+  // Characteristic variables of the union stencil, w[q][k] = L[k] . (U_q - ref)
+  // over the 7-vector (rho, mn, mt1, mt2, E, bt1, bt2), cell by cell.
+  T wch[UW][7];
 #pragma unroll
   for (int q = 0; q < UW; ++q) {
-    d[q][0] = cells[q].rho - ref.rho;
-    d[q][1] = cells[q].mx - ref.mx;
-    d[q][2] = cells[q].my - ref.my;
-    d[q][3] = cells[q].mz - ref.mz;
-    d[q][4] = cells[q].en - ref.en;
-    d[q][5] = cells[q].by - ref.by;
-    d[q][6] = cells[q].bz - ref.bz;
+    const St<T> cq = cell(q);
+    const T d[7] = {cq.rho - ref.rho, cq.mx - ref.mx, cq.my - ref.my, cq.mz - ref.mz,
+                    cq.en - ref.en,   cq.by - ref.by, cq.bz - ref.bz};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      T acc = T(0);
+#pragma unroll
+      for (int v = 0; v < 7; ++v) acc += e.L[k][v] * d[v];
+      wch[q][k] = acc;
+    }
   }
   T aL[7], aR[7];
 #pragma unroll
@@ -386,12 +393,7 @@ __host__ __device__ __forceinline__ void mp_char_face(const GridD& g, int dir, i
   for (int k = 0; k < 7; ++k) {
     T w[UW];
 #pragma unroll
-    for (int q = 0; q < UW; ++q) {
-      T acc = T(0);
-#pragma unroll
-      for (int v = 0; v < 7; ++v) acc += e.L[k][v] * d[q][v];
-      w[q] = acc;
-    }
+    for (int q = 0; q < UW; ++q) w[q] = wch[q][k];
     // Left state: cells 0..W-1 (centre H), right state: reversed cells 1..W.
     T sl[W], sr[W];
 #pragma unroll
@@ -467,6 +469,34 @@ __host__ __device__ __forceinline__ T physical_ez(T rho, T mx, T my, T bx, T by)
   const T vx = mx / rho;
   const T vy = my / rho;
   return vy * bx + vx * (-by);
+}
+
+// ---------------------------------------------------------------------------
+// Transverse quadrature (include/quasar/numerics/finite_volume_quadrature.hpp:
+// 12-68): a point value (or a Gauss average) from N samples with the exact
+// constant-sample shortcut, which returns the centre sample when all N agree.
+template <class T, int N>
+__host__ __device__ __forceinline__ T point_from_averages(const T* s, const double* w) {
+  bool constant = true;
+#pragma unroll
+  for (int t = 0; t < N; ++t) constant = constant && s[t] == s[N / 2];
+  if (constant) return s[N / 2];
+  T v = T(0);
+#pragma unroll
+  for (int t = 0; t < N; ++t) v += T(w[t]) * s[t];
+  return v;
+}
+
+// E_z of a recovered point state when the state is admissible (finite, rho > 0,
+// p > 0); false sends the cell to its cell-centre value (mhd_ct.hip:457-632).
+template <class T>
+__host__ __device__ __forceinline__ bool point_ez(const St<T>& st, T gm1, T& ez) {
+  const T p = pressure(st, gm1);
+  const bool adm = fin(st.rho) && st.rho > T(0) && fin(st.mx) && fin(st.my) && fin(st.mz) && fin(st.en) &&
+                   fin(st.bx) && fin(st.by) && fin(st.bz) && fin(p) && p > T(0);
+  if (!adm) return false;
+  ez = physical_ez(st.rho, st.mx, st.my, st.bx, st.by);
+  return true;
 }
 
 // ---------------------------------------------------------------------------

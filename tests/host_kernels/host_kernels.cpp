@@ -9,6 +9,50 @@
 
 using namespace qmhd;
 
+// High-order cell E_z of the cell-EMF kernel (separable tensor recovery: x
+// point values of each stencil row, then y point values, EOS, E_z and the
+// Gauss combine) on cells [0,nx] x [0,ny], from the collocated fields.
+template <int ORDER>
+static void cell_ez_tensor(const GridD& g, const QuadRows& rows, double gamma, const double* const* u8,
+                           const double* bxc, const double* byc, double* cell_ez) {
+  constexpr int H = ORDER == 5 ? 2 : 3;
+  constexpr int NQ = ORDER == 5 ? 3 : 4;
+  constexpr int TW = 2 * H + 1;
+  const double gm1 = gamma - 1.0;
+  const double* comp_plane[8] = {u8[0], u8[1], u8[2], u8[3], u8[4], bxc, byc, u8[7]};
+  for (int j = 0; j <= g.ny; ++j)
+    for (int i = 0; i <= g.nx; ++i) {
+      const std::size_t k = idx(g, i, j);
+      double ez = physical_ez(u8[0][k], u8[1][k], u8[2][k], bxc[k], byc[k]);
+      const bool in_range = i - H >= -g.ng && i + H <= g.nx + g.ng - 1 && j - H >= -g.ng && j + H <= g.ny + g.ng - 1;
+      bool ok = in_range;
+      double point[NQ][NQ];
+      for (int qx = 0; qx < NQ && ok; ++qx) {
+        const double* wx = ORDER == 5 ? rows.mp5_point[qx] : rows.mp7_point[qx];
+        double s[8][TW];
+        for (int c = 0; c < 8; ++c)
+          for (int r = 0; r < TW; ++r) {
+            double line[TW];
+            for (int t = 0; t < TW; ++t) line[t] = comp_plane[c][idx(g, i - H + t, j - H + r)];
+            s[c][r] = point_from_averages<double, TW>(line, wx);
+          }
+        for (int qy = 0; qy < NQ && ok; ++qy) {
+          const double* wy = ORDER == 5 ? rows.mp5_point[qy] : rows.mp7_point[qy];
+          St<double> st;
+          for (int c = 0; c < 8; ++c) set_comp(st, c, point_from_averages<double, TW>(s[c], wy));
+          ok = point_ez(st, gm1, point[qy][qx]);
+        }
+      }
+      if (ok) {
+        const double* wg = ORDER == 5 ? rows.mp5_gauss : rows.mp7_gauss;
+        double row[NQ];
+        for (int qy = 0; qy < NQ; ++qy) row[qy] = point_from_averages<double, NQ>(point[qy], wg);
+        ez = point_from_averages<double, NQ>(row, wg);
+      }
+      cell_ez[k] = ez;
+    }
+}
+
 extern "C" {
 
 struct HkGrid {
@@ -120,6 +164,17 @@ void hk_cell_ez_lo(const HkGrid* hg, const double* const* u8, const double* bxc,
       const std::size_t k = idx(g, i, j);
       cell_ez[k] = physical_ez(u8[0][k], u8[1][k], u8[2][k], bxc[k], byc[k]);
     }
+}
+
+void hk_cell_ez_tensor(const HkGrid* hg, int order, double gamma, const double* const* u8, const double* bxc,
+                       const double* byc, double* cell_ez) {
+  const GridD g = grid_of(hg);
+  QuadRows rows;
+  build_quad_rows(rows);
+  if (order == 5)
+    cell_ez_tensor<5>(g, rows, gamma, u8, bxc, byc, cell_ez);
+  else
+    cell_ez_tensor<7>(g, rows, gamma, u8, bxc, byc, cell_ez);
 }
 
 // Corner E_z on corners [0,nx] x [0,ny].

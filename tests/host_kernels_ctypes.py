@@ -32,6 +32,8 @@ class HostKernels:
         L.hk_eigen.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
         L.hk_hlld.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
         L.hk_cell_ez_lo.argtypes = [C.POINTER(OrcGrid), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hk_cell_ez_tensor.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]
         L.hk_corner.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_void_p, C.c_void_p]
         self.lib = L
@@ -71,4 +73,10 @@ class HostKernels:
         ey = np.ascontiguousarray(eqy, dtype=np.int32)
         self.lib.hk_corner(C.byref(g), order, _flags(flags), _planes(fx8), _planes(fy8), cell_ez.ctypes.data,
                            ex.ctypes.data, ey.ctypes.data, out.ctypes.data)
+        return out
+
+    def cell_ez_tensor(self, g, order, gamma, u8, bxc, byc):
+        out = np.zeros_like(u8[0])
+        self.lib.hk_cell_ez_tensor(C.byref(g), order, gamma, _planes(u8), bxc.ctypes.data, byc.ctypes.data,
+                                   out.ctypes.data)
         return out

@@ -202,3 +202,31 @@ def test_ct_corner_high_order_matches_oracle(oracle, hk, order, ng):
     mask[ng:ng + n + 1, ng:ng + n + 1] = True
     mask = mask.ravel()
     assert np.abs(ez - ref["ez"])[mask].max() <= 1e-14 * np.abs(ref["ez"]).max()
+
+
+@pytest.mark.parametrize("kind,order", [("ot", 7), ("ot", 5), ("rotor", 7), ("rotor", 5), ("brio", 5)])
+def test_cell_ez_tensor_recovery_matches_oracle(oracle, hk, kind, order):
+    """The cell-EMF kernel's separable 4x4 (3x3) tensor recovery, EOS and Gauss
+    combine against the oracle's cell-average E_z, per cell."""
+    g, gam, F, u = _case(oracle, kind)
+    ifs, _, _ = _faces(oracle, g, order, F, gam, u)
+    ref = oracle.ct_emf(g, order, 0, F, gam, u, ifs[0], ifs[1])
+    bxc, byc = hk.collocate(g, 0, 0, u[5]), hk.collocate(g, 1, 0, u[6])
+    got = hk.cell_ez_tensor(g, order, gam, u, bxc, byc)
+    nx, ny, ng = g.nx, g.ny, g.ng
+    hx = 1 if F[1] not in (0, 4) else 0
+    hy = 1 if F[3] not in (0, 4) else 0
+    mask = np.zeros((ny + 2 * ng, nx + 2 * ng), bool)
+    mask[ng:ng + ny + hy, ng:ng + nx + hx] = True
+    mask = mask.ravel()
+    scale = np.abs(ref["cell_ez"][mask]).max()
+    assert np.abs(got - ref["cell_ez"])[mask].max() <= 1e-14 * scale
+    # The corner's 1-D shortcuts read exact equalities of cell E_z along runs of
+    # 2H+2 cells; with the kernel's own cell E_z they must decide as the
+    # oracle's, so the corner E_z agrees to round-off.
+    _, fls, eq = _faces(oracle, g, order, F, gam, u)
+    ez = hk.corner(g, order, F, fls[0], fls[1], got, eq[0], eq[1])
+    cmask = np.zeros((ny + 2 * ng, nx + 2 * ng), bool)
+    cmask[ng:ng + ny + 1, ng:ng + nx + 1] = True
+    cmask = cmask.ravel()
+    assert np.abs(ez - ref["ez"])[cmask].max() <= 1e-14 * np.abs(ref["ez"][cmask]).max()
