@@ -174,37 +174,44 @@ KERNEL_SYMBOLS = {
 }
 
 
-def _counter_means(run_dir: str, counter: str):
-    """Mean per-dispatch value of `counter` for every kernel name in a rocprofv3
-    --pmc csv output directory."""
+def _counter_rows(run_dir: str, counter: str):
+    """(kernel name, value) of every dispatch's `counter` in a rocprofv3 --pmc
+    csv output directory."""
     import csv
     import glob
-    sums, counts = {}, {}
+    rows = []
     for path in glob.glob(os.path.join(run_dir, "**", "*counter_collection.csv"), recursive=True):
         with open(path, newline="") as f:
             for row in csv.DictReader(f):
-                if row.get("Counter_Name") != counter:
-                    continue
-                name = row["Kernel_Name"]
-                sums[name] = sums.get(name, 0.0) + float(row["Counter_Value"])
-                counts[name] = counts.get(name, 0) + 1
+                if row.get("Counter_Name") == counter:
+                    rows.append((row["Kernel_Name"], float(row["Counter_Value"])))
+    return rows
+
+
+def _means(rows):
+    sums, counts = {}, {}
+    for name, v in rows:
+        sums[name] = sums.get(name, 0.0) + v
+        counts[name] = counts.get(name, 0) + 1
     return {k: sums[k] / counts[k] for k in sums}
 
 
 def summarize_counters(fetch_dir: str, write_dir: str, n: int, probe_mib: int) -> dict:
     """HBM bytes per launch of each kernel kind.  FETCH_SIZE and WRITE_SIZE read
     KiB per dispatch; each is scaled by the factor that makes the copy probe
-    (probe_mib MiB read and written once) read exactly its size."""
-    fetch = _counter_means(fetch_dir, "FETCH_SIZE")
-    write = _counter_means(write_dir, "WRITE_SIZE")
-    probe = [k for k in fetch if "copy" in k.lower() or "elementwise" in k.lower()]
-    if not probe:
-        raise RuntimeError("no copy-probe kernel in the counter output")
+    (the largest copy dispatch: probe_mib MiB read and written once) count
+    exactly its size."""
+    frows = _counter_rows(fetch_dir, "FETCH_SIZE")
+    wrows = _counter_rows(write_dir, "WRITE_SIZE")
+    fprobe = max((v for k, v in frows if "copy" in k.lower()), default=0.0)
+    wprobe = max((v for k, v in wrows if "copy" in k.lower()), default=0.0)
+    if fprobe <= 0 or wprobe <= 0:
+        raise RuntimeError("no copy-probe dispatch in the counter output")
     probe_kib = probe_mib * 1024.0
-    fetch_scale = probe_kib / max(fetch[k] for k in probe)
-    write_scale = probe_kib / max(write.get(k, 0.0) for k in probe)
-    out = {"probe_kernel": max(probe, key=lambda k: fetch[k]), "fetch_scale": fetch_scale,
-           "write_scale": write_scale, "unit": "bytes per launch"}
+    fetch_scale, write_scale = probe_kib / fprobe, probe_kib / wprobe
+    fetch, write = _means(frows), _means(wrows)
+    out = {"probe_mib": probe_mib, "probe_fetch_kib": fprobe, "probe_write_kib": wprobe,
+           "fetch_scale": fetch_scale, "write_scale": write_scale, "unit": "bytes per launch"}
     for kind, sym in KERNEL_SYMBOLS.items():
         names = [k for k in fetch if sym in k]
         if not names:

@@ -67,14 +67,18 @@ __device__ __forceinline__ bool block_leader() { return threadIdx.x == 0 && thre
 
 // Internal energy q = E - K - M of a convex combination (the admissible set
 // rho > 0, q > 0 is convex).
+// Internal energy q = E - K - M of a convex combination (the admissible set
+// rho > 0, q > 0 is convex), in the reference's arithmetic
+// (mhd_reduce.hip:99-123): its sign decides the positivity controller.
 __device__ __forceinline__ double convex_q(const St<double>& a, const St<double>& b, double th) {
+#pragma clang fp contract(off)
   const double om = 1.0 - th;
   const double rho = om * a.rho + th * b.rho;
   if (!(rho > 0.0) || !fin(rho)) return -Lim<double>::inf();
   const double mx = om * a.mx + th * b.mx, my = om * a.my + th * b.my, mz = om * a.mz + th * b.mz;
   const double e = om * a.en + th * b.en;
   const double bx = om * a.bx + th * b.bx, by = om * a.by + th * b.by, bz = om * a.bz + th * b.bz;
-  const double q = (e - half_sq(mx, my, mz) / rho) - half_sq(bx, by, bz);
+  const double q = (e - kinetic_from_momentum(mx, my, mz, rho)) - half_squared_norm3(bx, by, bz);
   return fin(q) ? q : -Lim<double>::inf();
 }
 
@@ -322,7 +326,6 @@ __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_onl
   }
   const std::size_t ko = idx(g, i, j);
   const std::size_t k = idx(g, si, sj);
-  const T gm1 = gamma - T(1);
   St<T> baseL, baseR;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
@@ -367,7 +370,7 @@ __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_onl
     int first_bad = NQ;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const bool good = admissible_state(pl[q], gm1) && admissible_state(pr[q], gm1);
+      const bool good = admissible_state(pl[q], gamma) && admissible_state(pr[q], gamma);
       if (!good && first_bad == NQ) first_bad = q;
       ok = ok && good;
     }
@@ -473,7 +476,6 @@ __global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high,
   // Stencil bounds inside the padded domain (always true for the computed
   // cells when ng >= H + 1).
   const bool in_range = i - H >= -g.ng && i + H <= g.nx + g.ng - 1 && j - H >= -g.ng && j + H <= g.ny + g.ng - 1;
-  const T gm1 = gamma - T(1);
   T point[NQ][NQ];
   bool ok = in_range;
   for (int qx = 0; qx < NQ; ++qx) {
@@ -498,7 +500,7 @@ __global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high,
         St<T> st;
 #pragma unroll
         for (int c = 0; c < 8; ++c) set_comp(st, c, point_from_averages<T, TW>(s[c], wy));
-        ok = ok && point_ez(st, gm1, point[qy][qx]);
+        ok = ok && point_ez(st, gamma, point[qy][qx]);
       }
     }
     __syncthreads();

@@ -216,14 +216,166 @@ __host__ __device__ __forceinline__ T kinetic_from_momentum(T mx, T my, T mz, T 
   return (T(0.5) * q) * (q * ns);
 }
 
+// Primitive state and the reference's conversions (mhd_state.hpp:183-255):
+// velocities by division, kinetic and magnetic energies through the scaled
+// norms, no multiply-add contraction.  Reconstructed face states built from
+// these compare (L == R) exactly as the reference's do.
+template <class T>
+struct Prim {
+  T rho, vx, vy, vz, p, bx, by, bz;
+};
+
+template <class T>
+__host__ __device__ __forceinline__ T kinetic_from_velocity(T rho, T vx, T vy, T vz) {
+#pragma clang fp contract(off)
+  const T scale = fmax(__builtin_fabs(vx), fmax(__builtin_fabs(vy), __builtin_fabs(vz)));
+  if (scale == T(0)) return T(0);
+  const T xs = vx / scale, ys = vy / scale, zs = vz / scale;
+  const T q = __builtin_sqrt(rho) * scale;
+  const T ns = xs * xs + ys * ys + zs * zs;
+  return (T(0.5) * q) * (q * ns);
+}
+
+template <class T>
+__host__ __device__ __forceinline__ T ref_pressure(const St<T>& u, T gamma) {
+#pragma clang fp contract(off)
+  return (gamma - T(1)) * ((u.en - kinetic_from_momentum(u.mx, u.my, u.mz, u.rho)) - half_squared_norm3(u.bx, u.by, u.bz));
+}
+
+template <class T>
+__host__ __device__ __forceinline__ Prim<T> to_prim(const St<T>& u, T gamma) {
+#pragma clang fp contract(off)
+  return Prim<T>{u.rho, u.mx / u.rho, u.my / u.rho, u.mz / u.rho, ref_pressure(u, gamma), u.bx, u.by, u.bz};
+}
+
+template <class T>
+__host__ __device__ __forceinline__ St<T> to_cons(const Prim<T>& w, T gamma) {
+#pragma clang fp contract(off)
+  St<T> u;
+  u.rho = w.rho;
+  u.mx = w.rho * w.vx;
+  u.my = w.rho * w.vy;
+  u.mz = w.rho * w.vz;
+  u.bx = w.bx;
+  u.by = w.by;
+  u.bz = w.bz;
+  const T kinetic = kinetic_from_velocity(w.rho, w.vx, w.vy, w.vz);
+  const T magnetic = half_squared_norm3(w.bx, w.by, w.bz);
+  u.en = w.p / (gamma - T(1)) + kinetic + magnetic;
+  return u;
+}
+
 template <class T>
 struct Eig {
   T L[7][7];
   T R[7][7];  // R[var][wave]
 };
 
+// The characteristic basis about a reference state in factored form: the
+// scalars from which the primitive eigenvector rows Rp[k], Lp[k]
+// (var = rho, vn, vt1, vt2, p, bt1, bt2) and the maps M = dU/dW, M^-1
+// (mhd_eigensystem.hpp:333-379) are formed on demand.  The conserved
+// R = M Rp and L = Lp M^-1 are only materialised for the biorthogonality test,
+// which decides `identity`.
 template <class T>
-__host__ __device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<T>& e) {
+struct CharBasis {
+  T rho, sqrt_rho, vn, vt1, vt2, bt1, bt2, gm1, half_v2, inv_rho;
+  T alpha_f, alpha_s, beta1, beta2, sbn, rs2, cf, cs, qf, qs, Af, As, Nf, inv_a2, rho_a2;
+  bool identity;
+};
+
+// Primitive rows of wave k (unset entries zero).
+template <class T>
+__host__ __device__ __forceinline__ void basis_rows(const CharBasis<T>& B, int k, T* r, T* l) {
+#pragma clang fp contract(off)
+  const T rho = B.rho, sqrt_rho = B.sqrt_rho, alpha_f = B.alpha_f, alpha_s = B.alpha_s;
+  const T beta1 = B.beta1, beta2 = B.beta2, sbn = B.sbn, rs2 = B.rs2, cf = B.cf, cs = B.cs;
+  const T qf = B.qf, qs = B.qs, Af = B.Af, As = B.As, Nf = B.Nf, inv_a2 = B.inv_a2, rho_a2 = B.rho_a2;
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    r[v] = T(0);
+    l[v] = T(0);
+  }
+  if (k == 0) {
+    r[0] = rho * alpha_f; r[1] = -alpha_f * cf; r[2] = qs * beta1; r[3] = qs * beta2;
+    r[4] = alpha_f * rho_a2; r[5] = As * beta1; r[6] = As * beta2;
+    l[1] = -Nf * alpha_f * cf; l[2] = Nf * qs * beta1; l[3] = Nf * qs * beta2;
+    l[4] = Nf * alpha_f / rho; l[5] = Nf * As * beta1 / rho; l[6] = Nf * As * beta2 / rho;
+  } else if (k == 6) {
+    r[0] = rho * alpha_f; r[1] = alpha_f * cf; r[2] = -qs * beta1; r[3] = -qs * beta2;
+    r[4] = alpha_f * rho_a2; r[5] = As * beta1; r[6] = As * beta2;
+    l[1] = Nf * alpha_f * cf; l[2] = -Nf * qs * beta1; l[3] = -Nf * qs * beta2;
+    l[4] = Nf * alpha_f / rho; l[5] = Nf * As * beta1 / rho; l[6] = Nf * As * beta2 / rho;
+  } else if (k == 1) {
+    r[2] = -beta2 * rs2; r[3] = beta1 * rs2;
+    r[5] = -sbn * beta2 * rs2 * sqrt_rho; r[6] = sbn * beta1 * rs2 * sqrt_rho;
+    l[2] = -beta2 * rs2; l[3] = beta1 * rs2;
+    l[5] = -sbn * beta2 * rs2 / sqrt_rho; l[6] = sbn * beta1 * rs2 / sqrt_rho;
+  } else if (k == 5) {
+    r[2] = -beta2 * rs2; r[3] = beta1 * rs2;
+    r[5] = sbn * beta2 * rs2 * sqrt_rho; r[6] = -sbn * beta1 * rs2 * sqrt_rho;
+    l[2] = -beta2 * rs2; l[3] = beta1 * rs2;
+    l[5] = sbn * beta2 * rs2 / sqrt_rho; l[6] = -sbn * beta1 * rs2 / sqrt_rho;
+  } else if (k == 2) {
+    r[0] = rho * alpha_s; r[1] = -alpha_s * cs; r[2] = -qf * beta1; r[3] = -qf * beta2;
+    r[4] = alpha_s * rho_a2; r[5] = -Af * beta1; r[6] = -Af * beta2;
+    l[1] = -Nf * alpha_s * cs; l[2] = -Nf * qf * beta1; l[3] = -Nf * qf * beta2;
+    l[4] = Nf * alpha_s / rho; l[5] = -Nf * Af * beta1 / rho; l[6] = -Nf * Af * beta2 / rho;
+  } else if (k == 4) {
+    r[0] = rho * alpha_s; r[1] = alpha_s * cs; r[2] = qf * beta1; r[3] = qf * beta2;
+    r[4] = alpha_s * rho_a2; r[5] = -Af * beta1; r[6] = -Af * beta2;
+    l[1] = Nf * alpha_s * cs; l[2] = Nf * qf * beta1; l[3] = Nf * qf * beta2;
+    l[4] = Nf * alpha_s / rho; l[5] = -Nf * Af * beta1 / rho; l[6] = -Nf * Af * beta2 / rho;
+  } else {
+    r[0] = T(1);
+    l[0] = T(1); l[4] = -inv_a2;
+  }
+}
+
+// Conserved right column R[.][k] = M Rp[k] and left row L[k][.] = Lp[k] M^-1,
+// the non-zero entries of M and M^-1 in the reference's summation order.
+template <class T>
+__host__ __device__ __forceinline__ void basis_rcol(const CharBasis<T>& B, const T* r, T* out) {
+#pragma clang fp contract(off)
+  const T m41 = B.rho * B.vn, m42 = B.rho * B.vt1, m43 = B.rho * B.vt2, m44 = T(1) / B.gm1;
+  out[0] = r[0];
+  out[1] = B.vn * r[0] + B.rho * r[1];
+  out[2] = B.vt1 * r[0] + B.rho * r[2];
+  out[3] = B.vt2 * r[0] + B.rho * r[3];
+  out[4] = (((((B.half_v2 * r[0] + m41 * r[1]) + m42 * r[2]) + m43 * r[3]) + m44 * r[4]) + B.bt1 * r[5]) + B.bt2 * r[6];
+  out[5] = r[5];
+  out[6] = r[6];
+}
+template <class T>
+__host__ __device__ __forceinline__ void basis_lrow(const CharBasis<T>& B, const T* l, T* out) {
+#pragma clang fp contract(off)
+  const T gm1 = B.gm1, inv_rho = B.inv_rho;
+  const T i10 = -B.vn * inv_rho, i20 = -B.vt1 * inv_rho, i30 = -B.vt2 * inv_rho;
+  const T i40 = gm1 * B.half_v2, i41 = -gm1 * B.vn, i42 = -gm1 * B.vt1, i43 = -gm1 * B.vt2;
+  const T i45 = -gm1 * B.bt1, i46 = -gm1 * B.bt2;
+  out[0] = (((l[0] + l[1] * i10) + l[2] * i20) + l[3] * i30) + l[4] * i40;
+  out[1] = l[1] * inv_rho + l[4] * i41;
+  out[2] = l[2] * inv_rho + l[4] * i42;
+  out[3] = l[3] * inv_rho + l[4] * i43;
+  out[4] = l[4] * gm1;
+  out[5] = l[4] * i45 + l[5];
+  out[6] = l[4] * i46 + l[6];
+}
+
+// Seven-wave eigensystem about a reference state, +x normal, in the
+// conserved 7-vector (rho, mn, mt1, mt2, E, bt1, bt2) (Stone et al. 2008, App. A).
+//
+// Whether the characteristic basis is usable is decided by an absolute
+// 1024-epsilon test on L R - I (include/quasar/numerics/mhd_eigensystem.hpp:
+// 398-431).  Near degenerate states (zero transverse field with a ~ c_ax) the
+// residual sits close to that threshold, so the decision depends on how each
+// operation rounds: the scalars, the rows, M, M^-1 and the test follow the
+// reference's operation order term by term (its dense M Rp and Lp M^-1
+// products reduced to their structurally non-zero terms, in the same summation
+// order) with multiply-add contraction off, so the decision is that of the
+// reference arithmetic.
+template <class T>
+__host__ __device__ __forceinline__ void basis_build(const St<T>& s, T gamma, CharBasis<T>& B) {
 #pragma clang fp contract(off)
   // Primitive reference state (mhd_state.hpp:183-189, 228-239).
   const T rho = s.rho;
@@ -294,93 +446,132 @@ __host__ __device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<
   const T Nf = T(0.5) * inv_a2;
   const T rho_a2 = gamma * p;
 
-  // M = dU/dW and its inverse (mhd_eigensystem.hpp:333-379): only their
-  // non-zero entries enter the products below, in the reference's j order.
-  const T gm1 = gamma - T(1);
-  const T half_v2 = half_squared_norm3(vn, vt1, vt2);
-  const T m41 = rho * vn, m42 = rho * vt1, m43 = rho * vt2, m44 = T(1) / gm1;
-  const T inv_rho = T(1) / rho;
-  const T i10 = -vn * inv_rho, i20 = -vt1 * inv_rho, i30 = -vt2 * inv_rho;
-  const T i40 = gm1 * half_v2, i41 = -gm1 * vn, i42 = -gm1 * vt1, i43 = -gm1 * vt2;
-  const T i45 = -gm1 * bt1, i46 = -gm1 * bt2;
-  // Wave by wave: the primitive right/left eigenvector rows Rp[k][var] and
-  // Lp[k][var], var = (rho, vn, vt1, vt2, p, bt1, bt2), unset entries zero,
-  // are formed where they are transformed, so only the conserved L and R
-  // stay live.
+  B.rho = rho;
+  B.sqrt_rho = sqrt_rho;
+  B.vn = vn;
+  B.vt1 = vt1;
+  B.vt2 = vt2;
+  B.bt1 = bt1;
+  B.bt2 = bt2;
+  B.gm1 = gamma - T(1);
+  B.half_v2 = half_squared_norm3(vn, vt1, vt2);
+  B.inv_rho = T(1) / rho;
+  B.alpha_f = alpha_f;
+  B.alpha_s = alpha_s;
+  B.beta1 = beta1;
+  B.beta2 = beta2;
+  B.sbn = sbn;
+  B.rs2 = rs2;
+  B.cf = cf;
+  B.cs = cs;
+  B.qf = qf;
+  B.qs = qs;
+  B.Af = Af;
+  B.As = As;
+  B.Nf = Nf;
+  B.inv_a2 = inv_a2;
+  B.rho_a2 = rho_a2;
+  // Loss of biorthogonality -> component-wise limiting (identity basis).
+  T R[7][7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
-    T r[7] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-    T l[7] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-    if (k == 0) {
-      r[0] = rho * alpha_f; r[1] = -alpha_f * cf; r[2] = qs * beta1; r[3] = qs * beta2;
-      r[4] = alpha_f * rho_a2; r[5] = As * beta1; r[6] = As * beta2;
-      l[1] = -Nf * alpha_f * cf; l[2] = Nf * qs * beta1; l[3] = Nf * qs * beta2;
-      l[4] = Nf * alpha_f / rho; l[5] = Nf * As * beta1 / rho; l[6] = Nf * As * beta2 / rho;
-    } else if (k == 6) {
-      r[0] = rho * alpha_f; r[1] = alpha_f * cf; r[2] = -qs * beta1; r[3] = -qs * beta2;
-      r[4] = alpha_f * rho_a2; r[5] = As * beta1; r[6] = As * beta2;
-      l[1] = Nf * alpha_f * cf; l[2] = -Nf * qs * beta1; l[3] = -Nf * qs * beta2;
-      l[4] = Nf * alpha_f / rho; l[5] = Nf * As * beta1 / rho; l[6] = Nf * As * beta2 / rho;
-    } else if (k == 1) {
-      r[2] = -beta2 * rs2; r[3] = beta1 * rs2;
-      r[5] = -sbn * beta2 * rs2 * sqrt_rho; r[6] = sbn * beta1 * rs2 * sqrt_rho;
-      l[2] = -beta2 * rs2; l[3] = beta1 * rs2;
-      l[5] = -sbn * beta2 * rs2 / sqrt_rho; l[6] = sbn * beta1 * rs2 / sqrt_rho;
-    } else if (k == 5) {
-      r[2] = -beta2 * rs2; r[3] = beta1 * rs2;
-      r[5] = sbn * beta2 * rs2 * sqrt_rho; r[6] = -sbn * beta1 * rs2 * sqrt_rho;
-      l[2] = -beta2 * rs2; l[3] = beta1 * rs2;
-      l[5] = sbn * beta2 * rs2 / sqrt_rho; l[6] = -sbn * beta1 * rs2 / sqrt_rho;
-    } else if (k == 2) {
-      r[0] = rho * alpha_s; r[1] = -alpha_s * cs; r[2] = -qf * beta1; r[3] = -qf * beta2;
-      r[4] = alpha_s * rho_a2; r[5] = -Af * beta1; r[6] = -Af * beta2;
-      l[1] = -Nf * alpha_s * cs; l[2] = -Nf * qf * beta1; l[3] = -Nf * qf * beta2;
-      l[4] = Nf * alpha_s / rho; l[5] = -Nf * Af * beta1 / rho; l[6] = -Nf * Af * beta2 / rho;
-    } else if (k == 4) {
-      r[0] = rho * alpha_s; r[1] = alpha_s * cs; r[2] = qf * beta1; r[3] = qf * beta2;
-      r[4] = alpha_s * rho_a2; r[5] = -Af * beta1; r[6] = -Af * beta2;
-      l[1] = Nf * alpha_s * cs; l[2] = Nf * qf * beta1; l[3] = Nf * qf * beta2;
-      l[4] = Nf * alpha_s / rho; l[5] = -Nf * Af * beta1 / rho; l[6] = -Nf * Af * beta2 / rho;
-    } else {
-      r[0] = T(1);
-      l[0] = T(1); l[4] = -inv_a2;
-    }
-    e.R[0][k] = r[0];
-    e.R[1][k] = vn * r[0] + rho * r[1];
-    e.R[2][k] = vt1 * r[0] + rho * r[2];
-    e.R[3][k] = vt2 * r[0] + rho * r[3];
-    e.R[4][k] = (((((half_v2 * r[0] + m41 * r[1]) + m42 * r[2]) + m43 * r[3]) + m44 * r[4]) + bt1 * r[5]) + bt2 * r[6];
-    e.R[5][k] = r[5];
-    e.R[6][k] = r[6];
-    e.L[k][0] = (((l[0] + l[1] * i10) + l[2] * i20) + l[3] * i30) + l[4] * i40;
-    e.L[k][1] = l[1] * inv_rho + l[4] * i41;
-    e.L[k][2] = l[2] * inv_rho + l[4] * i42;
-    e.L[k][3] = l[3] * inv_rho + l[4] * i43;
-    e.L[k][4] = l[4] * gm1;
-    e.L[k][5] = l[4] * i45 + l[5];
-    e.L[k][6] = l[4] * i46 + l[6];
+    T r[7], l[7], col[7];
+    basis_rows(B, k, r, l);
+    basis_rcol(B, r, col);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) R[v][k] = col[v];
   }
-  // Loss of biorthogonality -> component-wise limiting (identity basis).
   bool usable = true;
 #pragma unroll
-  for (int i = 0; i < 7; ++i)
+  for (int i = 0; i < 7; ++i) {
+    T r[7], l[7], row[7];
+    basis_rows(B, i, r, l);
+    basis_lrow(B, l, row);
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
-      T dot = e.L[i][0] * e.R[0][j];
+      T dot = row[0] * R[0][j];
 #pragma unroll
-      for (int k = 1; k < 7; ++k) dot = dot + e.L[i][k] * e.R[k][j];
+      for (int k = 1; k < 7; ++k) dot = dot + row[k] * R[k][j];
       const T expect = (i == j) ? T(1) : T(0);
       usable = usable && fin(dot) && !(__builtin_fabs(dot - expect) > T(1024) * Lim<T>::eps);
     }
-  if (!usable) {
-#pragma unroll
-    for (int i = 0; i < 7; ++i)
-#pragma unroll
-      for (int j = 0; j < 7; ++j) {
-        e.L[i][j] = (i == j) ? T(1) : T(0);
-        e.R[i][j] = (i == j) ? T(1) : T(0);
-      }
   }
+  B.identity = !usable;
+}
+
+// The conserved L and R (identity when the basis is not usable).
+template <class T>
+__host__ __device__ __forceinline__ void eig_build(const St<T>& s, T gamma, Eig<T>& e) {
+  CharBasis<T> B;
+  basis_build(s, gamma, B);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    T r[7], l[7], col[7], row[7];
+    basis_rows(B, k, r, l);
+    basis_rcol(B, r, col);
+    basis_lrow(B, l, row);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      e.R[v][k] = B.identity ? T(v == k ? 1 : 0) : col[v];
+      e.L[k][v] = B.identity ? T(v == k ? 1 : 0) : row[v];
+    }
+  }
+}
+
+// Characteristic variables w = L d of a conserved 7-vector difference, as
+// Lp (M^-1 d).
+template <class T>
+__host__ __device__ __forceinline__ void basis_to_char(const CharBasis<T>& B, const T* d, T* w) {
+  if (B.identity) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) w[v] = d[v];
+    return;
+  }
+  const T gm1 = B.gm1, inv_rho = B.inv_rho;
+  T dp[7];
+  dp[0] = d[0];
+  dp[1] = (d[1] - B.vn * d[0]) * inv_rho;
+  dp[2] = (d[2] - B.vt1 * d[0]) * inv_rho;
+  dp[3] = (d[3] - B.vt2 * d[0]) * inv_rho;
+  dp[4] = gm1 * (((d[4] + B.half_v2 * d[0]) - (B.vn * d[1] + B.vt1 * d[2] + B.vt2 * d[3])) -
+                 (B.bt1 * d[5] + B.bt2 * d[6]));
+  dp[5] = d[5];
+  dp[6] = d[6];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    T r[7], l[7];
+    basis_rows(B, k, r, l);
+    T acc = T(0);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) acc += l[v] * dp[v];
+    w[k] = acc;
+  }
+}
+
+// Conserved 7-vector R w, as M (Rp^T w).
+template <class T>
+__host__ __device__ __forceinline__ void basis_from_char(const CharBasis<T>& B, const T* w, T* out) {
+  if (B.identity) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) out[v] = w[v];
+    return;
+  }
+  T ap[7] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    T r[7], l[7];
+    basis_rows(B, k, r, l);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) ap[v] += r[v] * w[k];
+  }
+  out[0] = ap[0];
+  out[1] = B.vn * ap[0] + B.rho * ap[1];
+  out[2] = B.vt1 * ap[0] + B.rho * ap[2];
+  out[3] = B.vt2 * ap[0] + B.rho * ap[3];
+  out[4] = B.half_v2 * ap[0] + B.rho * (B.vn * ap[1] + B.vt1 * ap[2] + B.vt2 * ap[3]) + ap[4] / B.gm1 +
+           B.bt1 * ap[5] + B.bt2 * ap[6];
+  out[5] = ap[5];
+  out[6] = ap[6];
 }
 
 // ---------------------------------------------------------------------------

@@ -274,20 +274,24 @@ __host__ __device__ __forceinline__ St<T> load_cell_lo(const GridD& g, const Sta
                T(face_to_cell_lo(g, u.c[5], 0, i, j)), T(face_to_cell_lo(g, u.c[6], 1, j, i)), T(u.c[7][k])};
 }
 
+// Reconstructed-state admissibility (mhd_reconstruct.hip:433-446).
 template <class T>
-__host__ __device__ __forceinline__ bool admissible_state(const St<T>& s, T gm1) {
+__host__ __device__ __forceinline__ bool admissible_state(const St<T>& s, T gamma) {
   if (!(fin(s.rho) && s.rho > T(0))) return false;
-  const T inv = T(1) / s.rho;
-  const T vx = s.mx * inv, vy = s.my * inv, vz = s.mz * inv;
-  const T p = pressure(s, gm1);
-  return fin(vx) && fin(vy) && fin(vz) && fin(p) && p > T(0) && fin(s.mx) && fin(s.my) && fin(s.mz) &&
+  const Prim<T> w = to_prim(s, gamma);
+  return fin(w.vx) && fin(w.vy) && fin(w.vz) && fin(w.p) && w.p > T(0) && fin(s.mx) && fin(s.my) && fin(s.mz) &&
          fin(s.en) && fin(s.bx) && fin(s.by) && fin(s.bz);
 }
 
+// Replace the normal field of a face state keeping its pressure: the energy
+// gains v^2/2 - bn^2/2 as one correctly rounded sum of the rounded products
+// (mhd_reconstruct.hip:141-153).
 template <class T>
 __host__ __device__ __forceinline__ void set_normal_b(St<T>& s, int dir, T v) {
+#pragma clang fp contract(off)
   T& bn = dir == 0 ? s.bx : s.by;
-  s.en = fma(-T(0.5) * bn, bn, fma(T(0.5) * v, v, s.en));
+  const double t[3] = {double(s.en), double(v * (T(0.5) * v)), double(bn * (-T(0.5) * bn))};
+  s.en = T(cr_sum<3>(t));
   bn = v;
 }
 
@@ -304,49 +308,34 @@ __host__ __device__ __forceinline__ int slope_drop(int c, int n, int lo, int hi)
   return 0;
 }
 
-// Second-order MUSCL-minmod in primitive variables (also the MP fallback).
+// Second-order MUSCL-minmod in primitive variables (also the MP fallback),
+// mhd_reconstruct.hip:96-120, 375-414, in the reference's arithmetic.
 template <class T>
-__host__ __device__ __noinline__ void muscl_face(const GridD& g, int dir, int i, int j, T gamma, int side_lo, int side_hi,
-                                        const StateView& u, const double* bxc, const double* byc, St<T>& ul,
-                                        St<T>& ur) {
+__host__ __device__ __noinline__ void muscl_face(const GridD& g, int dir, int i, int j, T gamma, int side_lo,
+                                                 int side_hi, const StateView& u, const double* bxc, const double* byc,
+                                                 St<T>& ul, St<T>& ur) {
+#pragma clang fp contract(off)
   const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
-  const T gm1 = gamma - T(1);
-  St<T> c[4];
-  for (int s = 0; s < 4; ++s) c[s] = load_cell<T>(g, u, bxc, byc, i + (s - 2) * di, j + (s - 2) * dj);
-  T w[4][8];
-  for (int s = 0; s < 4; ++s) {
-    const T inv = T(1) / c[s].rho;
-    w[s][0] = c[s].rho;
-    w[s][1] = c[s].mx * inv;
-    w[s][2] = c[s].my * inv;
-    w[s][3] = c[s].mz * inv;
-    w[s][4] = pressure(c[s], gm1);
-    w[s][5] = c[s].bx;
-    w[s][6] = c[s].by;
-    w[s][7] = c[s].bz;
-  }
+  Prim<T> w[4];
+  for (int s = 0; s < 4; ++s) w[s] = to_prim(load_cell<T>(g, u, bxc, byc, i + (s - 2) * di, j + (s - 2) * dj), gamma);
   const int n_dir = dir == 0 ? g.nx : g.ny;
   const int drop_m = slope_drop((dir == 0 ? i : j) - 1, n_dir, side_lo, side_hi);
   const int drop_p = slope_drop(dir == 0 ? i : j, n_dir, side_lo, side_hi);
-  T wl[8], wr[8];
-  for (int q = 0; q < 8; ++q) {
-    wl[q] = w[1][q] + T(0.5) * limited_slope(w[1][q] - w[0][q], w[2][q] - w[1][q], drop_m);
-    wr[q] = w[2][q] - T(0.5) * limited_slope(w[2][q] - w[1][q], w[3][q] - w[2][q], drop_p);
-  }
-  auto to_cons = [&](const T* p) {
-    St<T> s;
-    s.rho = p[0];
-    s.mx = p[0] * p[1];
-    s.my = p[0] * p[2];
-    s.mz = p[0] * p[3];
-    s.bx = p[5];
-    s.by = p[6];
-    s.bz = p[7];
-    s.en = p[4] / gm1 + p[0] * half_sq(p[1], p[2], p[3]) + half_sq(p[5], p[6], p[7]);
-    return s;
+  auto face = [&](const Prim<T>& wm, const Prim<T>& wc, const Prim<T>& wp, T sign, int drop) {
+    const T h = T(0.5) * sign;
+    Prim<T> f;
+    f.rho = wc.rho + h * limited_slope(wc.rho - wm.rho, wp.rho - wc.rho, drop);
+    f.vx = wc.vx + h * limited_slope(wc.vx - wm.vx, wp.vx - wc.vx, drop);
+    f.vy = wc.vy + h * limited_slope(wc.vy - wm.vy, wp.vy - wc.vy, drop);
+    f.vz = wc.vz + h * limited_slope(wc.vz - wm.vz, wp.vz - wc.vz, drop);
+    f.p = wc.p + h * limited_slope(wc.p - wm.p, wp.p - wc.p, drop);
+    f.bx = wc.bx + h * limited_slope(wc.bx - wm.bx, wp.bx - wc.bx, drop);
+    f.by = wc.by + h * limited_slope(wc.by - wm.by, wp.by - wc.by, drop);
+    f.bz = wc.bz + h * limited_slope(wc.bz - wm.bz, wp.bz - wc.bz, drop);
+    return f;
   };
-  ul = to_cons(wl);
-  ur = to_cons(wr);
+  ul = to_cons(face(w[0], w[1], w[2], T(1), drop_m), gamma);
+  ur = to_cons(face(w[1], w[2], w[3], T(-1), drop_p), gamma);
 }
 
 template <class T, int ORDER>
@@ -365,53 +354,37 @@ __host__ __device__ __forceinline__ void mp_char_face(const GridD& g, int dir, i
 #pragma unroll
     for (int c = 0; c < 8; ++c) set_comp(ref, c, T(0.5) * comp(cl, c) + T(0.5) * comp(cr, c));
   }
-  Eig<T> e;
-  eig_build(ref, gamma, e);
-  // Characteristic variables of the union stencil, w[q][k] = L[k] . (U_q - ref)
-  // over the 7-vector (rho, mn, mt1, mt2, E, bt1, bt2), cell by cell.
+  CharBasis<T> B;
+  basis_build(ref, gamma, B);
+  // Characteristic variables of the union stencil, w[q] = L (U_q - ref) over
+  // the 7-vector (rho, mn, mt1, mt2, E, bt1, bt2), cell by cell.
   T wch[UW][7];
 #pragma unroll
   for (int q = 0; q < UW; ++q) {
     const St<T> cq = cell(q);
     const T d[7] = {cq.rho - ref.rho, cq.mx - ref.mx, cq.my - ref.my, cq.mz - ref.mz,
                     cq.en - ref.en,   cq.by - ref.by, cq.bz - ref.bz};
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-      T acc = T(0);
-#pragma unroll
-      for (int v = 0; v < 7; ++v) acc += e.L[k][v] * d[v];
-      wch[q][k] = acc;
-    }
+    basis_to_char(B, d, wch[q]);
   }
-  T aL[7], aR[7];
-#pragma unroll
-  for (int v = 0; v < 7; ++v) {
-    aL[v] = T(0);
-    aR[v] = T(0);
-  }
+  T wl[7], wr[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
-    T w[UW];
-#pragma unroll
-    for (int q = 0; q < UW; ++q) w[q] = wch[q][k];
     // Left state: cells 0..W-1 (centre H), right state: reversed cells 1..W.
     T sl[W], sr[W];
 #pragma unroll
     for (int s = 0; s < W; ++s) {
-      sl[s] = w[s];
-      sr[s] = w[W - s];
+      sl[s] = wch[s][k];
+      sr[s] = wch[W - s][k];
     }
     const T fl = ORDER == 5 ? mp5_face(sl) : mp7_face(sl);
     const T fr = ORDER == 5 ? mp5_face(sr) : mp7_face(sr);
     constexpr int o = ORDER == 5 ? 0 : 1;
-    const T wl = mp_limit(fl, sl[o], sl[o + 1], sl[o + 2], sl[o + 3], sl[o + 4]);
-    const T wr = mp_limit(fr, sr[o], sr[o + 1], sr[o + 2], sr[o + 3], sr[o + 4]);
-#pragma unroll
-    for (int v = 0; v < 7; ++v) {
-      aL[v] += e.R[v][k] * wl;
-      aR[v] += e.R[v][k] * wr;
-    }
+    wl[k] = mp_limit(fl, sl[o], sl[o + 1], sl[o + 2], sl[o + 3], sl[o + 4]);
+    wr[k] = mp_limit(fr, sr[o], sr[o + 1], sr[o + 2], sr[o + 3], sr[o + 4]);
   }
+  T aL[7], aR[7];
+  basis_from_char(B, wl, aL);
+  basis_from_char(B, wr, aR);
   St<T> sL{ref.rho + aL[0], ref.mx + aL[1], ref.my + aL[2], ref.mz + aL[3], ref.en + aL[4], ref.bx + T(0), ref.by + aL[5], ref.bz + aL[6]};
   St<T> sR{ref.rho + aR[0], ref.mx + aR[1], ref.my + aR[2], ref.mz + aR[3], ref.en + aR[4], ref.bx + T(0), ref.by + aR[5], ref.bz + aR[6]};
   ul = rot(sL, dir);
@@ -427,7 +400,6 @@ __host__ __device__ __forceinline__ void reconstruct_face(const GridD& g, int di
                                                           int side_hi, const StateView& u, const double* bxc,
                                                           const double* byc, St<T>& ul, St<T>& ur) {
   const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
-  const T gm1 = gamma - T(1);
   if (ORDER <= 1) {
     ul = load_cell_lo<T>(g, u, i - di, j - dj);
     ur = load_cell_lo<T>(g, u, i, j);
@@ -435,13 +407,13 @@ __host__ __device__ __forceinline__ void reconstruct_face(const GridD& g, int di
     muscl_face<T>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
   } else {
     mp_char_face<T, ORDER>(g, dir, i, j, gamma, u, bxc, byc, ul, ur);
-    if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1))
+    if (!admissible_state(ul, gamma) || !admissible_state(ur, gamma))
       muscl_face<T>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
   }
   const T bn = T((dir == 0 ? u.c[5] : u.c[6])[idx(g, i, j)]);
   set_normal_b(ul, dir, bn);
   set_normal_b(ur, dir, bn);
-  if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1)) {
+  if (!admissible_state(ul, gamma) || !admissible_state(ur, gamma)) {
     if (ORDER <= 1) {
       ul = load_cell_lo<T>(g, u, i - di, j - dj);
       ur = load_cell_lo<T>(g, u, i, j);
@@ -452,7 +424,7 @@ __host__ __device__ __forceinline__ void reconstruct_face(const GridD& g, int di
     set_normal_b(ul, dir, bn);
     set_normal_b(ur, dir, bn);
   }
-  if (!admissible_state(ul, gm1) || !admissible_state(ur, gm1)) {
+  if (!admissible_state(ul, gamma) || !admissible_state(ur, gamma)) {
     const T nan = Lim<T>::nan();
     ul = St<T>{nan, nan, nan, nan, nan, nan, nan, nan};
     ur = ul;
@@ -490,11 +462,12 @@ __host__ __device__ __forceinline__ T point_from_averages(const T* s, const doub
 // E_z of a recovered point state when the state is admissible (finite, rho > 0,
 // p > 0); false sends the cell to its cell-centre value (mhd_ct.hip:457-632).
 template <class T>
-__host__ __device__ __forceinline__ bool point_ez(const St<T>& st, T gm1, T& ez) {
-  const T p = pressure(st, gm1);
-  const bool adm = fin(st.rho) && st.rho > T(0) && fin(st.mx) && fin(st.my) && fin(st.mz) && fin(st.en) &&
-                   fin(st.bx) && fin(st.by) && fin(st.bz) && fin(p) && p > T(0);
-  if (!adm) return false;
+__host__ __device__ __forceinline__ bool point_ez(const St<T>& st, T gamma, T& ez) {
+  if (!(fin(st.rho) && st.rho > T(0) && fin(st.mx) && fin(st.my) && fin(st.mz) && fin(st.en) && fin(st.bx) &&
+        fin(st.by) && fin(st.bz)))
+    return false;
+  const T p = ref_pressure(st, gamma);
+  if (!(fin(p) && p > T(0))) return false;
   ez = physical_ez(st.rho, st.mx, st.my, st.bx, st.by);
   return true;
 }

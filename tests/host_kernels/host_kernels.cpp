@@ -18,7 +18,6 @@ static void cell_ez_tensor(const GridD& g, const QuadRows& rows, double gamma, c
   constexpr int H = ORDER == 5 ? 2 : 3;
   constexpr int NQ = ORDER == 5 ? 3 : 4;
   constexpr int TW = 2 * H + 1;
-  const double gm1 = gamma - 1.0;
   const double* comp_plane[8] = {u8[0], u8[1], u8[2], u8[3], u8[4], bxc, byc, u8[7]};
   for (int j = 0; j <= g.ny; ++j)
     for (int i = 0; i <= g.nx; ++i) {
@@ -40,7 +39,7 @@ static void cell_ez_tensor(const GridD& g, const QuadRows& rows, double gamma, c
           const double* wy = ORDER == 5 ? rows.mp5_point[qy] : rows.mp7_point[qy];
           St<double> st;
           for (int c = 0; c < 8; ++c) set_comp(st, c, point_from_averages<double, TW>(s[c], wy));
-          ok = point_ez(st, gm1, point[qy][qx]);
+          ok = point_ez(st, gamma, point[qy][qx]);
         }
       }
       if (ok) {
