@@ -146,11 +146,11 @@ __global__ __launch_bounds__(256) void prep_kernel(GridD g, StateView u, double*
     const bool in_pad = i < g.nx + g.ng && j < g.ny + g.ng;
     double bxv = 0.0, byv = 0.0;
     if (in_pad) {
-      bxv = face_to_cell(g, u.c[5], 0, i, j);
-      byv = face_to_cell(g, u.c[6], 1, j, i);
+      // The outer ghost layers (one-sided closures) are filled by
+      // ghost_collocate_kernel; interior cells always take the centred stencil.
       const std::size_t k = idx(g, i, j);
-      bxc[k] = bxv;
-      byc[k] = byv;
+      if (face_to_cell_centred(g, u.c[5], 0, i, j, bxv)) bxc[k] = bxv;
+      if (face_to_cell_centred(g, u.c[6], 1, j, i, byv)) byc[k] = byv;
     }
     const bool interior = in_pad && i >= 0 && j >= 0 && i < g.nx && j < g.ny;
     if (interior) {
@@ -225,6 +225,27 @@ __global__ __launch_bounds__(256) void prep_kernel(GridD g, StateView u, double*
   }
 }
 
+// Collocation of the outer ghost layers: bx on the ng columns at each x end,
+// by on the ng rows at each y end (the one-sided closures live there).
+__global__ __launch_bounds__(256) void ghost_collocate_kernel(GridD g, const double* __restrict__ bxf,
+                                                              const double* __restrict__ byf,
+                                                              double* __restrict__ bxc, double* __restrict__ byc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int px = g.nx + 2 * g.ng, py = g.ny + 2 * g.ng;
+  const int nbx = 2 * g.ng * py;
+  const int nby = 2 * g.ng * px;
+  if (t < nbx) {
+    const int layer = t % (2 * g.ng), j = t / (2 * g.ng) - g.ng;
+    const int i = layer < g.ng ? layer - g.ng : g.nx + (layer - g.ng);
+    bxc[idx(g, i, j)] = face_to_cell(g, bxf, 0, i, j);
+  } else if (t < nbx + nby) {
+    const int r = t - nbx;
+    const int layer = r / px, i = r % px - g.ng;
+    const int j = layer < g.ng ? layer - g.ng : g.ny + (layer - g.ng);
+    byc[idx(g, i, j)] = face_to_cell(g, byf, 1, j, i);
+  }
+}
+
 // Stand-alone collocation (readback of bx/by and the launch-level ABI).
 __global__ __launch_bounds__(256) void collocate_kernel(GridD g, int axis, int co, const double* __restrict__ face,
                                                         double* __restrict__ cell) {
@@ -242,23 +263,9 @@ __global__ __launch_bounds__(256) void collocate_kernel(GridD g, int axis, int c
 // One thread per face normal to `dir`: dir 0 faces i in [0,nx], j in [-ng,ny+ng);
 // dir 1 faces i in [-ng,nx+ng), j in [0,ny].  Writes L (0..7) and R (8..15)
 // interface planes and the per-face "L == R" flag used by the CT 1-D shortcut.
-template <class T, int ORDER>
-__global__ __launch_bounds__(256) void reconstruct_kernel(GridD g, int dir, T gamma, int side_lo, int side_hi,
-                                                          StateView u, const double* __restrict__ bxc,
-                                                          const double* __restrict__ byc, IfaceView<T> out,
-                                                          int* __restrict__ eq_flag) {
-  const int li = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lj = blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = dir == 1 ? li - g.ng : li;
-  const int j = dir == 0 ? lj - g.ng : lj;
-  if (dir == 0) {
-    if (li > g.nx || lj >= g.ny + 2 * g.ng) return;
-  } else {
-    if (li >= g.nx + 2 * g.ng || lj > g.ny) return;
-  }
-  St<T> ul, ur;
-  reconstruct_face<T, ORDER>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
-  const std::size_t k = idx(g, i, j);
+template <class T>
+__device__ __forceinline__ void write_faces(const IfaceView<T>& out, int* eq_flag, std::size_t k, const St<T>& ul,
+                                            const St<T>& ur) {
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     out.c[c][k] = comp(ul, c);
@@ -268,6 +275,66 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(GridD g, int dir, T ga
 #pragma unroll
   for (int c = 0; c < 8; ++c) eq = eq && (comp(ul, c) == comp(ur, c));
   eq_flag[k] = eq ? 1 : 0;
+}
+
+// Face (i, j) of a reconstruction launch over dir 0 faces i in [0,nx],
+// j in [-ng, ny+ng) (dir 1 transposed); false outside.
+__device__ __forceinline__ bool recon_face_of(const GridD& g, int dir, int& i, int& j) {
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lj = blockIdx.y * blockDim.y + threadIdx.y;
+  i = dir == 1 ? li - g.ng : li;
+  j = dir == 0 ? lj - g.ng : lj;
+  if (dir == 0) return li <= g.nx && lj < g.ny + 2 * g.ng;
+  return li < g.nx + 2 * g.ng && lj <= g.ny;
+}
+
+// One thread per face normal to `dir`.  Writes L (0..7) and R (8..15)
+// interface planes and the per-face "L == R" flag used by the CT 1-D shortcut.
+// For MP5/MP7 this kernel takes the characteristic MP path only: a face whose
+// MP states (or their normal-field update) are not admissible is marked 2 in
+// the flag plane and redone, fallback chain included, by the fallback kernel.
+template <class T, int ORDER>
+__global__ __launch_bounds__(256) void reconstruct_kernel(GridD g, int dir, T gamma, int side_lo, int side_hi,
+                                                          StateView u, const double* __restrict__ bxc,
+                                                          const double* __restrict__ byc, IfaceView<T> out,
+                                                          int* __restrict__ eq_flag) {
+  int i, j;
+  if (!recon_face_of(g, dir, i, j)) return;
+  const std::size_t k = idx(g, i, j);
+  St<T> ul, ur;
+  if (ORDER >= 5) {
+    mp_char_face<T, ORDER>(g, dir, i, j, gamma, u, bxc, byc, ul, ur);
+    bool ok = admissible_state(ul, gamma) && admissible_state(ur, gamma);
+    if (ok) {
+      const T bn = T((dir == 0 ? u.c[5] : u.c[6])[k]);
+      set_normal_b(ul, dir, bn);
+      set_normal_b(ur, dir, bn);
+      ok = admissible_state(ul, gamma) && admissible_state(ur, gamma);
+    }
+    if (!ok) {
+      eq_flag[k] = 2;
+      return;
+    }
+  } else {
+    reconstruct_face<T, ORDER>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+  }
+  write_faces(out, eq_flag, k, ul, ur);
+}
+
+// The full fallback chain for the faces the MP kernel marked.
+template <class T, int ORDER>
+__global__ __launch_bounds__(256) void reconstruct_fallback_kernel(GridD g, int dir, T gamma, int side_lo,
+                                                                   int side_hi, StateView u,
+                                                                   const double* __restrict__ bxc,
+                                                                   const double* __restrict__ byc, IfaceView<T> out,
+                                                                   int* __restrict__ eq_flag) {
+  int i, j;
+  if (!recon_face_of(g, dir, i, j)) return;
+  const std::size_t k = idx(g, i, j);
+  if (eq_flag[k] != 2) return;
+  St<T> ul, ur;
+  reconstruct_face<T, ORDER>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+  write_faces(out, eq_flag, k, ul, ur);
 }
 
 // ---------------------------------------------------------------------------
@@ -1064,6 +1131,9 @@ void launch_prep(const GridD& g, const StateView& u, double* bxc, double* byc, c
   hipLaunchKernelGGL(prep_kernel, blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, dim3(64, 4 * kRowsPerThread)), b, 0,
                      stream, g, u, bxc, byc, base,
                      base_bxc, base_byc, theta_slot, theta_co, theta_both, live_checks, solver_owned, status);
+  const int nghost = 2 * g.ng * (g.ny + 2 * g.ng) + 2 * g.ng * (g.nx + 2 * g.ng);
+  hipLaunchKernelGGL(ghost_collocate_kernel, dim3((nghost + 255) / 256), dim3(256), 0, stream, g, u.c[5], u.c[6], bxc,
+                     byc);
 }
 
 void launch_collocate(const GridD& g, int axis, int co, const double* face, double* cell, hipStream_t stream) {
@@ -1090,9 +1160,13 @@ void launch_reconstruct(const GridD& g, int dir, int order, double gamma, const 
       break;
     case 5:
       hipLaunchKernelGGL((reconstruct_kernel<double, 5>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq);
+      hipLaunchKernelGGL((reconstruct_fallback_kernel<double, 5>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc,
+                         byc, out, eq);
       break;
     default:
       hipLaunchKernelGGL((reconstruct_kernel<double, 7>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq);
+      hipLaunchKernelGGL((reconstruct_fallback_kernel<double, 7>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc,
+                         byc, out, eq);
       break;
   }
 }

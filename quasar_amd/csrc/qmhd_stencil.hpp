@@ -213,6 +213,38 @@ __host__ __device__ __noinline__ inline double one_sided_average(const double* s
   return cr_sum_loop(terms, n);
 }
 
+// Centred high-order cell average (8/6/4/2 taps) when the centred stencil
+// lies inside the padded grid (every interior cell); false in the outer ghost
+// layers, whose values need the one-sided closure.
+__host__ __device__ __forceinline__ bool face_to_cell_centred(const GridD& g, const double* f, int axis, int cell,
+                                                              int fixed, double& out) {
+  const int n = axis == 0 ? g.nx : g.ny;
+  const int width = colloc_width(n + 2 * g.ng, g.ng);
+  const int lo = -g.ng, hi = n + g.ng - 1;
+  if (width == 1) {
+    out = axis_face(g, f, axis, cell, fixed);
+    return true;
+  }
+  const int start_c = cell - (width / 2 - 1);
+  if (!(start_c >= lo && start_c + width - 1 <= hi)) return false;
+  double s[8];
+  for (int k = 0; k < width; ++k) s[k] = axis_face(g, f, axis, start_c + k, fixed);
+  if (width == 8) {
+    const double w[8] = {-191.0, 1879.0, -9531.0, 68323.0, 68323.0, -9531.0, 1879.0, -191.0};
+    out = rational_stencil<8>(w, s, 120960.0);
+  } else if (width == 6) {
+    const double w[6] = {11.0, -93.0, 802.0, 802.0, -93.0, 11.0};
+    out = rational_stencil<6>(w, s, 1440.0);
+  } else if (width == 4) {
+    const double w[4] = {-1.0, 13.0, 13.0, -1.0};
+    out = rational_stencil<4>(w, s, 24.0);
+  } else {
+    const double w[2] = {1.0, 1.0};
+    out = rational_stencil<2>(w, s, 2.0);
+  }
+  return true;
+}
+
 // High-order cell average (8/6/4/2 taps, one-sided closures in the outer
 // ghosts), mhd_staggering.hpp:175-236.
 __host__ __device__ inline double face_to_cell(const GridD& g, const double* f, int axis, int cell, int fixed) {
