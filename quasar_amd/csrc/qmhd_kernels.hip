@@ -237,12 +237,12 @@ __global__ __launch_bounds__(256) void ghost_collocate_kernel(GridD g, const dou
   if (t < nbx) {
     const int layer = t % (2 * g.ng), j = t / (2 * g.ng) - g.ng;
     const int i = layer < g.ng ? layer - g.ng : g.nx + (layer - g.ng);
-    bxc[idx(g, i, j)] = face_to_cell(g, bxf, 0, i, j);
+    bxc[idx(g, i, j)] = face_to_cell_ghost(g, bxf, 0, i, j);
   } else if (t < nbx + nby) {
     const int r = t - nbx;
     const int layer = r / px, i = r % px - g.ng;
     const int j = layer < g.ng ? layer - g.ng : g.ny + (layer - g.ng);
-    byc[idx(g, i, j)] = face_to_cell(g, byf, 1, j, i);
+    byc[idx(g, i, j)] = face_to_cell_ghost(g, byf, 1, j, i);
   }
 }
 
@@ -291,50 +291,75 @@ __device__ __forceinline__ bool recon_face_of(const GridD& g, int dir, int& i, i
 // One thread per face normal to `dir`.  Writes L (0..7) and R (8..15)
 // interface planes and the per-face "L == R" flag used by the CT 1-D shortcut.
 // For MP5/MP7 this kernel takes the characteristic MP path only: a face whose
-// MP states (or their normal-field update) are not admissible is marked 2 in
-// the flag plane and redone, fallback chain included, by the fallback kernel.
+// MP states (or their normal-field update) are not admissible is appended to
+// `fb_list` (one atomic per wave) and redone, fallback chain included, by the
+// fallback kernel.
 template <class T, int ORDER>
 __global__ __launch_bounds__(256) void reconstruct_kernel(GridD g, int dir, T gamma, int side_lo, int side_hi,
                                                           StateView u, const double* __restrict__ bxc,
                                                           const double* __restrict__ byc, IfaceView<T> out,
-                                                          int* __restrict__ eq_flag) {
+                                                          int* __restrict__ eq_flag,
+                                                          unsigned long long* __restrict__ fb_count,
+                                                          unsigned long long* __restrict__ fb_list) {
   int i, j;
-  if (!recon_face_of(g, dir, i, j)) return;
-  const std::size_t k = idx(g, i, j);
+  const bool valid = recon_face_of(g, dir, i, j);
+  const std::size_t k = valid ? idx(g, i, j) : 0;
   St<T> ul, ur;
-  if (ORDER >= 5) {
-    mp_char_face<T, ORDER>(g, dir, i, j, gamma, u, bxc, byc, ul, ur);
-    bool ok = admissible_state(ul, gamma) && admissible_state(ur, gamma);
-    if (ok) {
-      const T bn = T((dir == 0 ? u.c[5] : u.c[6])[k]);
-      set_normal_b(ul, dir, bn);
-      set_normal_b(ur, dir, bn);
-      ok = admissible_state(ul, gamma) && admissible_state(ur, gamma);
+  bool need = false;
+  if (valid) {
+    if (ORDER >= 5) {
+      mp_char_face<T, ORDER>(g, dir, i, j, gamma, u, bxc, byc, ul, ur);
+      bool ok = admissible_state(ul, gamma) && admissible_state(ur, gamma);
+      if (ok) {
+        const T bn = T((dir == 0 ? u.c[5] : u.c[6])[k]);
+        set_normal_b(ul, dir, bn);
+        set_normal_b(ur, dir, bn);
+        ok = admissible_state(ul, gamma) && admissible_state(ur, gamma);
+      }
+      need = !ok;
+    } else {
+      reconstruct_face<T, ORDER>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
     }
-    if (!ok) {
-      eq_flag[k] = 2;
-      return;
-    }
-  } else {
-    reconstruct_face<T, ORDER>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
   }
+  if (ORDER >= 5) {
+    const unsigned long long mask = __ballot(need);
+    if (mask != 0ull) {
+      const int lane = static_cast<int>(__lane_id());
+      const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(fb_count, static_cast<unsigned long long>(__popcll(mask)));
+      base = __shfl(base, leader, 64);
+      if (need) fb_list[base + __popcll(mask & ((1ull << lane) - 1ull))] = k;
+    }
+  }
+  if (!valid || need) return;
   write_faces(out, eq_flag, k, ul, ur);
 }
 
-// The full fallback chain for the faces the MP kernel marked.
+// The full fallback chain for the faces the MP kernel listed; a small
+// grid-stride launch.
 template <class T, int ORDER>
 __global__ __launch_bounds__(256) void reconstruct_fallback_kernel(GridD g, int dir, T gamma, int side_lo,
                                                                    int side_hi, StateView u,
                                                                    const double* __restrict__ bxc,
                                                                    const double* __restrict__ byc, IfaceView<T> out,
-                                                                   int* __restrict__ eq_flag) {
-  int i, j;
-  if (!recon_face_of(g, dir, i, j)) return;
-  const std::size_t k = idx(g, i, j);
-  if (eq_flag[k] != 2) return;
-  St<T> ul, ur;
-  reconstruct_face<T, ORDER>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
-  write_faces(out, eq_flag, k, ul, ur);
+                                                                   int* __restrict__ eq_flag,
+                                                                   const unsigned long long* __restrict__ fb_count,
+                                                                   const unsigned long long* __restrict__ fb_list) {
+  const unsigned long long n = *fb_count;
+  for (unsigned long long t = blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
+    const std::size_t k = static_cast<std::size_t>(fb_list[t]);
+    const int i = static_cast<int>(k % static_cast<std::size_t>(g.pitch)) - g.ng;
+    const int j = static_cast<int>(k / static_cast<std::size_t>(g.pitch)) - g.ng;
+    St<T> ul, ur;
+    reconstruct_face<T, ORDER>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+    write_faces(out, eq_flag, k, ul, ur);
+  }
+}
+
+__global__ void counter_reset_kernel(unsigned long long* __restrict__ c) {
+  if (threadIdx.x == 0) c[0] = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -1144,29 +1169,36 @@ void launch_collocate(const GridD& g, int axis, int co, const double* face, doub
 
 void launch_reconstruct(const GridD& g, int dir, int order, double gamma, const Flags4& flags, const StateView& u,
                         const double* bxc, const double* byc, const IfaceView<double>& out, int* eq,
-                        hipStream_t stream) {
+                        unsigned long long* fb_count, unsigned long long* fb_list, hipStream_t stream) {
   const dim3 b(64, 4);
   const int lx = dir == 0 ? g.nx + 1 : g.nx + 2 * g.ng;
   const int ly = dir == 0 ? g.ny + 2 * g.ng : g.ny + 1;
   const int slo = dir == 0 ? flags.side[0] : flags.side[2];
   const int shi = dir == 0 ? flags.side[1] : flags.side[3];
   const dim3 grid = blocks2d(lx, ly, b);
+  const dim3 fb_grid(64), fb_block(256);
   switch (order) {
     case 1:
-      hipLaunchKernelGGL((reconstruct_kernel<double, 1>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq);
+      hipLaunchKernelGGL((reconstruct_kernel<double, 1>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq,
+                         fb_count, fb_list);
       break;
     case 2:
-      hipLaunchKernelGGL((reconstruct_kernel<double, 2>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq);
+      hipLaunchKernelGGL((reconstruct_kernel<double, 2>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq,
+                         fb_count, fb_list);
       break;
     case 5:
-      hipLaunchKernelGGL((reconstruct_kernel<double, 5>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq);
-      hipLaunchKernelGGL((reconstruct_fallback_kernel<double, 5>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc,
-                         byc, out, eq);
+      hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb_count);
+      hipLaunchKernelGGL((reconstruct_kernel<double, 5>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq,
+                         fb_count, fb_list);
+      hipLaunchKernelGGL((reconstruct_fallback_kernel<double, 5>), fb_grid, fb_block, 0, stream, g, dir, gamma, slo, shi,
+                         u, bxc, byc, out, eq, fb_count, fb_list);
       break;
     default:
-      hipLaunchKernelGGL((reconstruct_kernel<double, 7>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq);
-      hipLaunchKernelGGL((reconstruct_fallback_kernel<double, 7>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc,
-                         byc, out, eq);
+      hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb_count);
+      hipLaunchKernelGGL((reconstruct_kernel<double, 7>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq,
+                         fb_count, fb_list);
+      hipLaunchKernelGGL((reconstruct_fallback_kernel<double, 7>), fb_grid, fb_block, 0, stream, g, dir, gamma, slo, shi,
+                         u, bxc, byc, out, eq, fb_count, fb_list);
       break;
   }
 }

@@ -72,7 +72,7 @@ void launch_prep(const GridD& g, const StateView& u, double* bxc, double* byc, c
 void launch_collocate(const GridD& g, int axis, int co, const double* face, double* cell, hipStream_t stream);
 void launch_reconstruct(const GridD& g, int dir, int order, double gamma, const Flags4& flags, const StateView& u,
                         const double* bxc, const double* byc, const IfaceView<double>& out, int* eq,
-                        hipStream_t stream);
+                        unsigned long long* fb_count, unsigned long long* fb_list, hipStream_t stream);
 void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4& flags, double gamma,
                  const IfaceView<double>& in, const FluxView<double>& flux, double* rate, hipStream_t stream);
 void launch_cfl_reduce(const GridD& g, const double* rate_x, const double* rate_y, unsigned long long* max_rate,

@@ -265,6 +265,23 @@ __host__ __device__ __forceinline__ St<T> to_cons(const Prim<T>& w, T gamma) {
   return u;
 }
 
+// Sign of the reference pressure (p > 0 and finite), decided with a fast
+// evaluation and a rounding margin: when the fast value is farther from zero
+// than 32 epsilon of the magnitudes it cancels, the reference form has the
+// same sign; otherwise the reference form decides.
+template <class T>
+__host__ __device__ __forceinline__ bool pressure_positive(const St<T>& u, T gamma) {
+  const T kin = T(0.5) * (u.mx * u.mx + u.my * u.my + u.mz * u.mz) / u.rho;
+  const T mag = T(0.5) * (u.bx * u.bx + u.by * u.by + u.bz * u.bz);
+  const T gm1 = gamma - T(1);
+  const T pf = gm1 * ((u.en - kin) - mag);
+  const T margin = T(32) * Lim<T>::eps * gm1 * (__builtin_fabs(u.en) + kin + mag);
+  if (fin(pf) && fin(margin) && pf > margin) return true;
+  if (fin(pf) && fin(margin) && pf < -margin) return false;
+  const T p = ref_pressure(u, gamma);
+  return fin(p) && p > T(0);
+}
+
 template <class T>
 struct Eig {
   T L[7][7];

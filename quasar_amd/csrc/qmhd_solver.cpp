@@ -327,6 +327,8 @@ class Solver {
     eqy_ = DBuf(n);
     cell_ez_ = DBuf(n);
     ez_ = DBuf(n);
+    fb_list_ = DBuf(n);
+    fb_count_ = DBuf(1);
     QMHD_HIP(hipMalloc(&status_, sizeof(StepStatus)));
     QMHD_HIP(hipMemset(status_, 0, sizeof(StepStatus)));
     QMHD_HIP(hipHostMalloc(&host_status_, sizeof(StepStatus), hipHostMallocDefault));
@@ -450,14 +452,14 @@ class Solver {
     (void)want_rates;
     clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_, [&] {
       launch_reconstruct(g_, 0, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface(ip),
-                         reinterpret_cast<int*>(eqx_.p), stream_);
+                         reinterpret_cast<int*>(eqx_.p), fb_count(), fb_list(), stream_);
     });
     clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
       launch_flux(g_, 0, order, hll, field_, gamma_, iface(ip), fluxv(fxp), rate_x_.p, stream_);
     });
     clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_, [&] {
       launch_reconstruct(g_, 1, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface(ip),
-                         reinterpret_cast<int*>(eqy_.p), stream_);
+                         reinterpret_cast<int*>(eqy_.p), fb_count(), fb_list(), stream_);
     });
     clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
       launch_flux(g_, 1, order, hll, field_, gamma_, iface(ip), fluxv(fyp), rate_y_.p, stream_);
@@ -703,6 +705,10 @@ class Solver {
   std::unique_ptr<Register> backup_;
   DBuf iface_[16], fx_[8], fy_[8];
   DBuf rate_x_, rate_y_, eqx_, eqy_, cell_ez_, ez_;
+  // Faces whose MP states need the fallback chain (list + count).
+  DBuf fb_list_, fb_count_;
+  unsigned long long* fb_list() { return reinterpret_cast<unsigned long long*>(fb_list_.p); }
+  unsigned long long* fb_count() { return reinterpret_cast<unsigned long long*>(fb_count_.p); }
   StepStatus* status_{nullptr};
   StepStatus* host_status_{nullptr};
   int cache_order_{-1};
@@ -889,7 +895,9 @@ int qmhd_reconstruct(const qmhd_grid* g, int dir, int order, const int32_t* flag
     qmhd::launch_collocate(d, 0, 0, u8[5], s.plane(0), st(stream));
     qmhd::launch_collocate(d, 1, 0, u8[6], s.plane(1), st(stream));
     qmhd::launch_reconstruct(d, dir, order, gamma, qmhd::flags_of(flags4), u, s.plane(0), s.plane(1),
-                             qmhd::iface(out16), reinterpret_cast<int*>(s.plane(2)), st(stream));
+                             qmhd::iface(out16), reinterpret_cast<int*>(s.plane(2)),
+                             reinterpret_cast<unsigned long long*>(s.tail()),
+                             reinterpret_cast<unsigned long long*>(s.plane(3)), st(stream));
     qmhd::hip_check(hipGetLastError(), "reconstruct");
   });
 }

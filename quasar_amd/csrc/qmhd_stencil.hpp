@@ -97,6 +97,42 @@ __host__ __device__ __forceinline__ double cr_sum(const double* t) {
   return hi == 0.0 ? 0.0 : hi;
 }
 
+// Correctly rounded sum of N finite terms, certified fast path: the terms are
+// accumulated in double-double (Ogita-Rump-Oishi Sum2: s plus the exact
+// two-sum errors summed in e), whose distance from the exact sum is at most
+// (N-1)^2 eps^2 sum|t| (bounded here by 512 eps^2 sum|t|).  s + e = r + rr
+// exactly with r = fl(s + e); the exact sum lies within that bound of r + rr,
+// so r is its correctly rounded value whenever |rr| plus the bound stays below
+// half the spacing to r's neighbours (a quarter ulp when |r| is a power of
+// two, whose lower neighbour is half an ulp away).  Otherwise -- exact ties,
+// heavy cancellation, zero -- the full expansion decides.  Same value as
+// cr_sum<N>.
+template <int N>
+__host__ __device__ __forceinline__ double cr_sum_fast(const double* t) {
+#pragma clang fp contract(off)
+  double s = t[0], e = 0.0, mag = __builtin_fabs(t[0]);
+#pragma unroll
+  for (int k = 1; k < N; ++k) {
+    const double a = s, b = t[k];
+    s = a + b;
+    const double bv = s - a;
+    e = e + ((a - (s - bv)) + (b - bv));
+    mag = mag + __builtin_fabs(b);
+  }
+  const double r = s + e;
+  const double bv = r - s;
+  const double rr = (s - (r - bv)) + (e - bv);
+  const double eps = 2.220446049250313080847e-16;
+  const double bound = 512.0 * eps * eps * mag;
+  if (r != 0.0 && __builtin_isfinite(r) && __builtin_isfinite(mag)) {
+    int ex;
+    const double mant = __builtin_frexp(r, &ex);  // |mant| in [0.5, 1), ulp(r) = 2^(ex-53)
+    const double half_gap = __builtin_ldexp(1.0, __builtin_fabs(mant) == 0.5 ? ex - 55 : ex - 54);
+    if (__builtin_fabs(rr) + bound < half_gap) return r;
+  }
+  return cr_sum<N>(t);
+}
+
 // Integer-weight stencil sum_k w_k s_k / D with the numerator correctly rounded
 // from the error-free products (mhd_staggering.hpp:59-70): the value of the
 // reference's exact rational collocation.
@@ -119,7 +155,7 @@ __host__ __device__ __forceinline__ double rational_stencil(const double* w, con
     t[2 * k] = hi;
     t[2 * k + 1] = fma(w[k], s[k], -hi);
   }
-  return cr_sum<2 * W>(t) / denom;
+  return cr_sum_fast<2 * W>(t) / denom;
 }
 
 // Correctly rounded sum as cr_sum<N>, with the expansion compacted in a loop
@@ -245,6 +281,65 @@ __host__ __device__ __forceinline__ bool face_to_cell_centred(const GridD& g, co
   return true;
 }
 
+// The one-sided closure of width W with the expansion kept in registers (the
+// same correctly rounded value as one_sided_average); for the ghost-layer
+// kernel, where register use does not limit anything else.
+template <int W>
+__host__ __device__ __forceinline__ double one_sided_unrolled(const double* s, int rel) {
+#pragma clang fp contract(off)
+  const double xq[4] = {0.0694318442029737124, 0.3300094782075718676, 0.6699905217924281324,
+                        0.9305681557970262876};
+  const double wq[4] = {0.1739274225687269287, 0.3260725774312730713, 0.3260725774312730713,
+                        0.1739274225687269287};
+  double bary[W];
+  double c = 1.0;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    bary[k] = (k % 2 == 0) ? c : -c;
+    c = c * double(W - 1 - k) / double(k + 1);
+  }
+  bool finite = true;
+#pragma unroll
+  for (int k = 0; k < W; ++k) finite = finite && __builtin_isfinite(s[k]);
+  double terms[4 * W];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double x = double(rel) + xq[q];
+    double dt[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) dt[k] = bary[k] / (x - double(k));
+    const double den = cr_sum<W>(dt);
+    const double wscaled = wq[q] / 1.0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) terms[q * W + k] = (((wscaled * bary[k]) * s[k]) / (x - double(k))) / den;
+  }
+  if (!finite) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4 * W; ++k) acc = acc + terms[k];
+    return acc;
+  }
+  return cr_sum<4 * W>(terms);
+}
+
+// face_to_cell with the register-resident one-sided closures.
+__host__ __device__ inline double face_to_cell_ghost(const GridD& g, const double* f, int axis, int cell, int fixed) {
+  double v;
+  if (face_to_cell_centred(g, f, axis, cell, fixed, v)) return v;
+  const int n = axis == 0 ? g.nx : g.ny;
+  const int width = colloc_width(n + 2 * g.ng, g.ng);
+  const int lo = -g.ng, hi = n + g.ng - 1;
+  int start = cell - (width / 2 - 1);
+  if (start < lo) start = lo;
+  if (start + width - 1 > hi) start = hi - width + 1;
+  double s[8];
+  for (int k = 0; k < width; ++k) s[k] = axis_face(g, f, axis, start + k, fixed);
+  if (width == 8) return one_sided_unrolled<8>(s, cell - start);
+  if (width == 6) return one_sided_unrolled<6>(s, cell - start);
+  if (width == 4) return one_sided_unrolled<4>(s, cell - start);
+  return one_sided_unrolled<2>(s, cell - start);
+}
+
 // High-order cell average (8/6/4/2 taps, one-sided closures in the outer
 // ghosts), mhd_staggering.hpp:175-236.
 __host__ __device__ inline double face_to_cell(const GridD& g, const double* f, int axis, int cell, int fixed) {
@@ -306,13 +401,14 @@ __host__ __device__ __forceinline__ St<T> load_cell_lo(const GridD& g, const Sta
                T(face_to_cell_lo(g, u.c[5], 0, i, j)), T(face_to_cell_lo(g, u.c[6], 1, j, i)), T(u.c[7][k])};
 }
 
-// Reconstructed-state admissibility (mhd_reconstruct.hip:433-446).
+// Reconstructed-state admissibility (mhd_reconstruct.hip:433-446): finite
+// components and primitive velocities, rho > 0, reference pressure > 0.
 template <class T>
 __host__ __device__ __forceinline__ bool admissible_state(const St<T>& s, T gamma) {
   if (!(fin(s.rho) && s.rho > T(0))) return false;
-  const Prim<T> w = to_prim(s, gamma);
-  return fin(w.vx) && fin(w.vy) && fin(w.vz) && fin(w.p) && w.p > T(0) && fin(s.mx) && fin(s.my) && fin(s.mz) &&
-         fin(s.en) && fin(s.bx) && fin(s.by) && fin(s.bz);
+  if (!(fin(s.mx) && fin(s.my) && fin(s.mz) && fin(s.en) && fin(s.bx) && fin(s.by) && fin(s.bz))) return false;
+  if (!(fin(s.mx / s.rho) && fin(s.my / s.rho) && fin(s.mz / s.rho))) return false;
+  return pressure_positive(s, gamma);
 }
 
 // Replace the normal field of a face state keeping its pressure: the energy
@@ -498,8 +594,7 @@ __host__ __device__ __forceinline__ bool point_ez(const St<T>& st, T gamma, T& e
   if (!(fin(st.rho) && st.rho > T(0) && fin(st.mx) && fin(st.my) && fin(st.mz) && fin(st.en) && fin(st.bx) &&
         fin(st.by) && fin(st.bz)))
     return false;
-  const T p = ref_pressure(st, gamma);
-  if (!(fin(p) && p > T(0))) return false;
+  if (!pressure_positive(st, gamma)) return false;
   ez = physical_ez(st.rho, st.mx, st.my, st.bx, st.by);
   return true;
 }
@@ -530,8 +625,8 @@ __host__ __device__ __forceinline__ double edge_interp(int order, const double* 
   return e[0] == e[1] ? e[0] : (e[0] + e[1]) / 2.0;
 }
 
-template <class T>
-__host__ __device__ inline double corner_ez(const GridD& g, int order, const Flags4& m, const FluxView<T>& fx,
+template <class T, bool INTERIOR>
+__host__ __device__ __forceinline__ double corner_ez_impl(const GridD& g, int order, const Flags4& m, const FluxView<T>& fx,
                                             const FluxView<T>& fy, const double* cell_ez, const int* eqx,
                                             const int* eqy, int i, int j) {
   const bool conducting = (i == 0 && (m.side[0] == 2 || m.side[0] == 3)) || (i == g.nx && m.side[1] == 2) ||
@@ -540,8 +635,9 @@ __host__ __device__ inline double corner_ez(const GridD& g, int order, const Fla
   const int lo = order >= 7 ? -4 : order >= 5 ? -3 : -1;
   const int nt = order >= 7 ? 8 : order >= 5 ? 6 : 2;
   const int rh = order >= 7 ? 3 : order >= 5 ? 2 : 0;  // transverse recovery half-width
-  auto mi = [&](int q) { return face_index(q, g.nx, m.side[0], m.side[1]); };
-  auto mj = [&](int q) { return face_index(q, g.ny, m.side[2], m.side[3]); };
+  // Away from the physical boundary every index maps to itself.
+  auto mi = [&](int q) { return INTERIOR ? q : face_index(q, g.nx, m.side[0], m.side[1]); };
+  auto mj = [&](int q) { return INTERIOR ? q : face_index(q, g.ny, m.side[2], m.side[3]); };
   double e[8];
   for (int t = 0; t < nt; ++t) e[t] = -static_cast<double>(fx.c[6][idx(g, mi(i), mj(j + lo + t))]);
   const double ex = edge_interp(order, e);
@@ -594,4 +690,19 @@ __host__ __device__ inline double corner_ez(const GridD& g, int order, const Fla
   return r;
 }
 
+
+template <class T>
+__host__ __device__ inline double corner_ez(const GridD& g, int order, const Flags4& m, const FluxView<T>& fx,
+                                            const FluxView<T>& fy, const double* cell_ez, const int* eqx,
+                                            const int* eqy, int i, int j) {
+  const int lo = order >= 7 ? -4 : order >= 5 ? -3 : -1;
+  const int nt = order >= 7 ? 8 : order >= 5 ? 6 : 2;
+  const int rh = order >= 7 ? 3 : order >= 5 ? 2 : 0;
+  // Every face, cell and flag index the corner touches lies in
+  // [i+lo-rh, i+lo+nt-1+rh] x [j+lo-rh, j+lo+nt-1+rh].
+  const bool interior = i + lo - rh >= 0 && i + lo + nt - 1 + rh < g.nx && j + lo - rh >= 0 &&
+                        j + lo + nt - 1 + rh < g.ny && i > 0 && j > 0;
+  return interior ? corner_ez_impl<T, true>(g, order, m, fx, fy, cell_ez, eqx, eqy, i, j)
+                  : corner_ez_impl<T, false>(g, order, m, fx, fy, cell_ez, eqx, eqy, i, j);
+}
 }  // namespace qmhd

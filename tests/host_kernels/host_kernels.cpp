@@ -88,6 +88,15 @@ void hk_collocate(const HkGrid* hg, int axis, int co, const double* face, double
     }
 }
 
+// Ghost-layer collocation as the ghost kernel computes it (register-resident
+// one-sided closures) on every padded cell.
+void hk_collocate_ghost(const HkGrid* hg, int axis, const double* face, double* cell) {
+  const GridD g = grid_of(hg);
+  for (int j = -g.ng; j < g.ny + g.ng; ++j)
+    for (int i = -g.ng; i < g.nx + g.ng; ++i)
+      cell[idx(g, i, j)] = axis == 0 ? face_to_cell_ghost(g, face, 0, i, j) : face_to_cell_ghost(g, face, 1, j, i);
+}
+
 // Interface states of every face the reconstruction kernel covers, from the
 // collocated cell fields bxc/byc (every padded cell).
 void hk_reconstruct_from(const HkGrid* hg, int dir, int order, const int32_t* flags4, double gamma,

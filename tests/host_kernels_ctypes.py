@@ -25,6 +25,7 @@ class HostKernels:
         ensure_built()
         L = C.CDLL(str(HOST_SO))
         L.hk_collocate.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.hk_collocate_ghost.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_void_p, C.c_void_p]
         L.hk_reconstruct.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_int, C.c_void_p, C.c_double,
                                      C.c_void_p, C.c_void_p]
         L.hk_reconstruct_from.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_int, C.c_void_p, C.c_double,
@@ -41,6 +42,11 @@ class HostKernels:
     def collocate(self, g, axis, co, face):
         out = np.zeros_like(face)
         self.lib.hk_collocate(C.byref(g), axis, co, face.ctypes.data, out.ctypes.data)
+        return out
+
+    def collocate_ghost(self, g, axis, face):
+        out = np.zeros_like(face)
+        self.lib.hk_collocate_ghost(C.byref(g), axis, face.ctypes.data, out.ctypes.data)
         return out
 
     def reconstruct(self, g, d, order, flags, gamma, u8):

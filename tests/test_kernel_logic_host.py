@@ -26,17 +26,35 @@ def hk():
     return HostKernels()
 
 
-@pytest.mark.parametrize("ng,nx,ny", [(4, 13, 9), (3, 11, 7), (2, 9, 6), (4, 3, 5)])
-def test_collocation_bitwise(oracle, hk, ng, nx, ny):
+@pytest.mark.parametrize("ng,nx,ny,data", [(4, 13, 9, "normal"), (3, 11, 7, "normal"), (2, 9, 6, "normal"),
+                                           (4, 3, 5, "normal"), (4, 11, 7, "integers"), (3, 11, 7, "dyadic"),
+                                           (4, 11, 7, "thirds"), (2, 11, 7, "tiny")])
+def test_collocation_bitwise(oracle, hk, ng, nx, ny, data):
+    """Correctly rounded collocation, including the certified fast path of the
+    exact sum: integer and dyadic samples make exact rounding ties, repeated
+    thirds make near-ties, subnormal-range samples make the bound vanish."""
     rng = np.random.default_rng(ng * 100 + nx)
     g = oracle.grid(nx, ny, ng)
     npad = (nx + 2 * ng) * (ny + 2 * ng)
-    face = rng.normal(size=npad) * 10.0 ** rng.integers(-3, 3, size=npad)
-    face[rng.random(npad) < 0.1] = 0.0
-    face[:5] = 1.0 / 3.0  # exactly representable ties in the stencil sums
+    if data == "integers":
+        face = rng.integers(-1000, 1000, npad).astype(float)
+    elif data == "dyadic":
+        face = rng.integers(-8, 8, npad) * 2.0 ** rng.integers(-30, 30, npad)
+    elif data == "thirds":
+        face = np.full(npad, 1.0 / 3.0)
+        face[rng.integers(0, npad, 7)] = 2.0 / 3.0
+    elif data == "tiny":
+        face = rng.normal(size=npad) * 1e-300
+    else:
+        face = rng.normal(size=npad) * 10.0 ** rng.integers(-3, 3, size=npad)
+        face[rng.random(npad) < 0.1] = 0.0
+        face[:5] = 1.0 / 3.0
     for axis in (0, 1):
+        ghost = hk.collocate_ghost(g, axis, face)
         for co in (0, 1):
             got = hk.collocate(g, axis, co, face)
+            if co == 0:
+                np.testing.assert_array_equal(ghost, got)
             for j in range(-ng, ny + ng):
                 for i in range(-ng, nx + ng):
                     k = (i + ng) + (nx + 2 * ng) * (j + ng)
