@@ -77,6 +77,7 @@ def parse():
                     help="bound on the CPU baseline's timed work")
     ap.add_argument("--cpu-n", type=int, default=256, help="cells per side of the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the timed region, copy a buffer of this size once (HBM traffic calibration "
                          "for counter runs)")
@@ -99,13 +100,14 @@ def orszag_tang(n: int):
     return spec
 
 
-def make_solver(n: int):
+def make_solver(n: int, arithmetic: str = "fp64"):
     from quasar_amd import mhd
     spec = orszag_tang(n)
     cfg = mhd.MhdConfig()
     cfg.grid = spec.grid
     cfg.gamma = spec.gamma
     cfg.reconstruction = "mp7"
+    cfg.arithmetic = arithmetic
     cfg.cfl = 0.4
     cfg.boundary.set_fluid_all("periodic")
     cfg.boundary.set_field_all("periodic")
@@ -114,6 +116,41 @@ def make_solver(n: int):
     for name, plane in state.items():
         solver.seed_state(name, plane)
     return solver
+
+
+STATE_NAMES = ("rho", "mx", "my", "mz", "energy", "bx_face", "by_face", "bz_cell")
+GROUPS = ((0,), (1, 2, 3), (1, 2, 3), (1, 2, 3), (4,), (5, 6, 7), (5, 6, 7), (5, 6, 7))
+
+
+def interior_state(solver, n: int):
+    import numpy as np
+    return [np.asarray(solver.state_component_to_host(nm)).reshape(n + 2 * NG, n + 2 * NG)[NG:NG + n, NG:NG + n]
+            for nm in STATE_NAMES]
+
+
+def rel_l1_by_variable(got, ref):
+    """Per-variable relative L1; momentum and field components against the L1
+    norm of their vector (mz and bz are round-off residue in 2-D problems)."""
+    import numpy as np
+    out = {}
+    for c, nm in enumerate(STATE_NAMES):
+        denom = max(float(np.abs(ref[k]).sum()) for k in GROUPS[c])
+        out[nm] = float(np.abs(got[c] - ref[c]).sum()) / denom
+    return out
+
+
+def timed_run(solver, steps: int):
+    import torch
+    solver.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done, sim_t = solver.run(steps)
+    solver.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if done != steps:
+        raise RuntimeError(f"run stopped after {done} of {steps} steps")
+    return wall, sim_t
 
 
 def units(kind: str, n: int) -> int:
@@ -288,15 +325,8 @@ def main():
     print(f"bench: warmup {args.warmup} steps done", file=sys.stderr, flush=True)
 
     solver.set_profiling(True)
-    solver.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    done, sim_t = solver.run(args.steps)
-    solver.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    if done != args.steps:
-        raise RuntimeError(f"run stopped after {done} of {args.steps} steps")
+    wall, sim_t = timed_run(solver, args.steps)
+    done = args.steps
     roof, shares = roofline(solver, n)
     solver.set_profiling(False)
     print(f"bench: {done} steps, wall {wall:.3f} s, kernels {sum(shares.values()) / 1e3:.3f} s", file=sys.stderr,
@@ -323,6 +353,24 @@ def main():
         "kernel_ms": shares,
         "roofline": roof,
     }
+    if not args.no_fp32:
+        # The FP32 form on the same workload and step count, and its distance
+        # from the FP64 run after the same number of steps.
+        print("bench: FP32 form", file=sys.stderr, flush=True)
+        ref_state = interior_state(solver, n)
+        del solver
+        f32 = make_solver(n, "fp32")
+        if args.warmup > 0:
+            f32.run(args.warmup)
+        wall32, sim32 = timed_run(f32, args.steps)
+        line["forms"] = {"fp32": {
+            "value": round(n * n * args.steps / wall32, 1),
+            "ms_per_step": round(wall32 / args.steps * 1e3, 4),
+            "speedup_vs_fp64": round(wall / wall32, 3),
+            "sim_time": sim32,
+            "rel_l1_vs_fp64": rel_l1_by_variable(interior_state(f32, n), ref_state),
+        }}
+        del f32
     if not args.no_cpu_baseline:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_seconds)

@@ -155,6 +155,12 @@ typedef struct {
   char reconstruction[32], riemann[32], integrator[32], ct[32], positivity[32], geometry[32];
   char fluid[4][32];
   char field[4][32];
+  /* Extension (no reference counterpart): arithmetic of the face work, "fp64"
+   * (default, also "") or "fp32".  In "fp32" the interface states, their
+   * reconstruction, the Riemann fluxes and the cell E_z quadrature run in
+   * FP32; the state, collocation, positivity recompute, CFL rates and
+   * reduction, corner E_z, update and conserved sums stay FP64. */
+  char arithmetic[32];
 } qmhd_solver_config;
 
 int qmhd_solver_create(const qmhd_solver_config* cfg, qmhd_solver** out);

@@ -41,7 +41,7 @@ class QSolverConfig(C.Structure):
     _fields_ = [("grid", QGrid), ("gamma", D), ("cfl", D), ("rho_floor", D), ("p_floor", D),
                 ("reconstruction", Name32), ("riemann", Name32), ("integrator", Name32),
                 ("ct", Name32), ("positivity", Name32), ("geometry", Name32),
-                ("fluid", Name32 * 4), ("field", Name32 * 4)]
+                ("fluid", Name32 * 4), ("field", Name32 * 4), ("arithmetic", Name32)]
 
 
 EXPORTS = [

@@ -386,7 +386,7 @@ __device__ __forceinline__ void speed_parts(const St<T>& s, int dir, T gamma, T&
   const T bn = dir == 0 ? s.bx : s.by;
   const T bt = dir == 0 ? s.by : s.bx;
   fast = fast_speed_x(s.rho, p, bn, bt, s.bz, gamma);
-  vel = __builtin_fabs(dir == 0 ? s.mx : s.my) / s.rho;
+  vel = qabs(dir == 0 ? s.mx : s.my) / s.rho;
   if (!(fin(vel) && fin(fast))) {
     vel = Lim<T>::inf();
     fast = Lim<T>::inf();
@@ -1167,8 +1167,9 @@ void launch_collocate(const GridD& g, int axis, int co, const double* face, doub
                      cell);
 }
 
+template <class T>
 void launch_reconstruct(const GridD& g, int dir, int order, double gamma, const Flags4& flags, const StateView& u,
-                        const double* bxc, const double* byc, const IfaceView<double>& out, int* eq,
+                        const double* bxc, const double* byc, const IfaceView<T>& out, int* eq,
                         unsigned long long* fb_count, unsigned long long* fb_list, hipStream_t stream) {
   const dim3 b(64, 4);
   const int lx = dir == 0 ? g.nx + 1 : g.nx + 2 * g.ng;
@@ -1177,49 +1178,52 @@ void launch_reconstruct(const GridD& g, int dir, int order, double gamma, const 
   const int shi = dir == 0 ? flags.side[1] : flags.side[3];
   const dim3 grid = blocks2d(lx, ly, b);
   const dim3 fb_grid(64), fb_block(256);
+  const T gamma_t = T(gamma);
   switch (order) {
     case 1:
-      hipLaunchKernelGGL((reconstruct_kernel<double, 1>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq,
+      hipLaunchKernelGGL((reconstruct_kernel<T, 1>), grid, b, 0, stream, g, dir, gamma_t, slo, shi, u, bxc, byc, out, eq,
                          fb_count, fb_list);
       break;
     case 2:
-      hipLaunchKernelGGL((reconstruct_kernel<double, 2>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq,
+      hipLaunchKernelGGL((reconstruct_kernel<T, 2>), grid, b, 0, stream, g, dir, gamma_t, slo, shi, u, bxc, byc, out, eq,
                          fb_count, fb_list);
       break;
     case 5:
       hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb_count);
-      hipLaunchKernelGGL((reconstruct_kernel<double, 5>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq,
+      hipLaunchKernelGGL((reconstruct_kernel<T, 5>), grid, b, 0, stream, g, dir, gamma_t, slo, shi, u, bxc, byc, out, eq,
                          fb_count, fb_list);
-      hipLaunchKernelGGL((reconstruct_fallback_kernel<double, 5>), fb_grid, fb_block, 0, stream, g, dir, gamma, slo, shi,
+      hipLaunchKernelGGL((reconstruct_fallback_kernel<T, 5>), fb_grid, fb_block, 0, stream, g, dir, gamma_t, slo, shi,
                          u, bxc, byc, out, eq, fb_count, fb_list);
       break;
     default:
       hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb_count);
-      hipLaunchKernelGGL((reconstruct_kernel<double, 7>), grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, out, eq,
+      hipLaunchKernelGGL((reconstruct_kernel<T, 7>), grid, b, 0, stream, g, dir, gamma_t, slo, shi, u, bxc, byc, out, eq,
                          fb_count, fb_list);
-      hipLaunchKernelGGL((reconstruct_fallback_kernel<double, 7>), fb_grid, fb_block, 0, stream, g, dir, gamma, slo, shi,
+      hipLaunchKernelGGL((reconstruct_fallback_kernel<T, 7>), fb_grid, fb_block, 0, stream, g, dir, gamma_t, slo, shi,
                          u, bxc, byc, out, eq, fb_count, fb_list);
       break;
   }
 }
 
+template <class T>
 void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4& flags, double gamma,
-                 const IfaceView<double>& in, const FluxView<double>& flux, double* rate, hipStream_t stream) {
+                 const IfaceView<T>& in, const FluxView<T>& flux, double* rate, hipStream_t stream) {
+  const T gm = T(gamma);
   const dim3 b(64, 4);
   const dim3 grid = blocks2d(g.nx + 1, g.ny + 1, b);
   const int periodic = dir == 0 ? (flags.side[0] == 0 && flags.side[1] == 0) : (flags.side[2] == 0 && flags.side[3] == 0);
   switch (order) {
     case 1:
-      hipLaunchKernelGGL((flux_kernel<double, 1>), grid, b, 0, stream, g, dir, hll_only, periodic, gamma, in, flux, rate);
+      hipLaunchKernelGGL((flux_kernel<T, 1>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate);
       break;
     case 2:
-      hipLaunchKernelGGL((flux_kernel<double, 2>), grid, b, 0, stream, g, dir, hll_only, periodic, gamma, in, flux, rate);
+      hipLaunchKernelGGL((flux_kernel<T, 2>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate);
       break;
     case 5:
-      hipLaunchKernelGGL((flux_kernel<double, 5>), grid, b, 0, stream, g, dir, hll_only, periodic, gamma, in, flux, rate);
+      hipLaunchKernelGGL((flux_kernel<T, 5>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate);
       break;
     default:
-      hipLaunchKernelGGL((flux_kernel<double, 7>), grid, b, 0, stream, g, dir, hll_only, periodic, gamma, in, flux, rate);
+      hipLaunchKernelGGL((flux_kernel<T, 7>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate);
       break;
   }
 }
@@ -1231,6 +1235,7 @@ void launch_cfl_reduce(const GridD& g, const double* rate_x, const double* rate_
                      rate_y, max_rate);
 }
 
+template <class T>
 void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamma, const StateView& u,
                      const double* bxc, const double* byc, double* cell_ez, hipStream_t stream) {
   const int pxh = flags.side[1] != 0 && flags.side[1] != 4;
@@ -1239,9 +1244,9 @@ void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamm
     const dim3 b(kEmfBx, kEmfBy);
     const dim3 grid = blocks2d(g.nx + 1, g.ny + 1, b);
     if (order == 5)
-      hipLaunchKernelGGL((cell_emf_kernel<double, 5>), grid, b, 0, stream, g, pxh, pyh, gamma, u, bxc, byc, cell_ez);
+      hipLaunchKernelGGL((cell_emf_kernel<T, 5>), grid, b, 0, stream, g, pxh, pyh, T(gamma), u, bxc, byc, cell_ez);
     else
-      hipLaunchKernelGGL((cell_emf_kernel<double, 7>), grid, b, 0, stream, g, pxh, pyh, gamma, u, bxc, byc, cell_ez);
+      hipLaunchKernelGGL((cell_emf_kernel<T, 7>), grid, b, 0, stream, g, pxh, pyh, T(gamma), u, bxc, byc, cell_ez);
   } else {
     const dim3 b(64, 4);
     hipLaunchKernelGGL(cell_emf_lo_kernel, blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, pxh, pyh,
@@ -1249,19 +1254,20 @@ void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamm
   }
 }
 
-void launch_corner(const GridD& g, int order, const Flags4& flags, const FluxView<double>& fx,
-                   const FluxView<double>& fy, const double* cell_ez, const int* eqx, const int* eqy, double* ez,
-                   hipStream_t stream) {
+template <class T>
+void launch_corner(const GridD& g, int order, const Flags4& flags, const FluxView<T>& fx, const FluxView<T>& fy,
+                   const double* cell_ez, const int* eqx, const int* eqy, double* ez, hipStream_t stream) {
   const dim3 b(64, 4);
-  hipLaunchKernelGGL((corner_kernel<double>), blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, order, flags, fx, fy,
+  hipLaunchKernelGGL((corner_kernel<T>), blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, order, flags, fx, fy,
                      cell_ez, eqx, eqy, ez);
 }
 
+template <class T>
 void launch_update(const GridD& g, double a, double b_, double cw, const double* dt_dev, const StateView& un,
-                   const StateView& us, const FluxView<double>& fx, const FluxView<double>& fy, const double* ez,
+                   const StateView& us, const FluxView<T>& fx, const FluxView<T>& fy, const double* ez,
                    const StateOut& out, hipStream_t stream) {
   const dim3 b(64, 4);
-  hipLaunchKernelGGL((update_kernel<double>), blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, a, b_, cw, dt_dev, un,
+  hipLaunchKernelGGL((update_kernel<T>), blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, a, b_, cw, dt_dev, un,
                      us, fx, fy, ez, out);
 }
 
@@ -1315,5 +1321,24 @@ void launch_cell_sums(const GridD& g, const StateView& u, double* partials, int 
 hipError_t upload_quad_rows(const QuadRows& rows) {
   return hipMemcpyToSymbol(HIP_SYMBOL(c_rows), &rows, sizeof(QuadRows));
 }
+
+
+// FP64 and FP32 forms of the face-level launchers.
+#define QMHD_INSTANTIATE(T)                                                                                        \
+  template void launch_reconstruct<T>(const GridD&, int, int, double, const Flags4&, const StateView&,            \
+                                      const double*, const double*, const IfaceView<T>&, int*, unsigned long long*, \
+                                      unsigned long long*, hipStream_t);                                          \
+  template void launch_flux<T>(const GridD&, int, int, int, const Flags4&, double, const IfaceView<T>&,            \
+                               const FluxView<T>&, double*, hipStream_t);                                         \
+  template void launch_cell_emf<T>(const GridD&, int, const Flags4&, double, const StateView&, const double*,     \
+                                   const double*, double*, hipStream_t);                                          \
+  template void launch_corner<T>(const GridD&, int, const Flags4&, const FluxView<T>&, const FluxView<T>&,        \
+                                 const double*, const int*, const int*, double*, hipStream_t);                    \
+  template void launch_update<T>(const GridD&, double, double, double, const double*, const StateView&,           \
+                                 const StateView&, const FluxView<T>&, const FluxView<T>&, const double*,         \
+                                 const StateOut&, hipStream_t);
+QMHD_INSTANTIATE(double)
+QMHD_INSTANTIATE(float)
+#undef QMHD_INSTANTIATE
 
 }  // namespace qmhd

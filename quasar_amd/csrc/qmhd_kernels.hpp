@@ -70,20 +70,27 @@ void launch_prep(const GridD& g, const StateView& u, double* bxc, double* byc, c
                  const double* base_bxc, const double* base_byc, int theta_slot, int theta_co, int theta_both,
                  int live_checks, int solver_owned, StepStatus* status, hipStream_t stream);
 void launch_collocate(const GridD& g, int axis, int co, const double* face, double* cell, hipStream_t stream);
+// The face-level launchers are instantiated for T = double (the FP64 form)
+// and T = float (the FP32 form: interface states, fluxes and their arithmetic
+// in FP32; state, collocation, cell/corner E_z, rates and the update in FP64).
+template <class T>
 void launch_reconstruct(const GridD& g, int dir, int order, double gamma, const Flags4& flags, const StateView& u,
-                        const double* bxc, const double* byc, const IfaceView<double>& out, int* eq,
+                        const double* bxc, const double* byc, const IfaceView<T>& out, int* eq,
                         unsigned long long* fb_count, unsigned long long* fb_list, hipStream_t stream);
+template <class T>
 void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4& flags, double gamma,
-                 const IfaceView<double>& in, const FluxView<double>& flux, double* rate, hipStream_t stream);
+                 const IfaceView<T>& in, const FluxView<T>& flux, double* rate, hipStream_t stream);
 void launch_cfl_reduce(const GridD& g, const double* rate_x, const double* rate_y, unsigned long long* max_rate,
                        hipStream_t stream);
+template <class T>
 void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamma, const StateView& u,
                      const double* bxc, const double* byc, double* cell_ez, hipStream_t stream);
-void launch_corner(const GridD& g, int order, const Flags4& flags, const FluxView<double>& fx,
-                   const FluxView<double>& fy, const double* cell_ez, const int* eqx, const int* eqy, double* ez,
-                   hipStream_t stream);
+template <class T>
+void launch_corner(const GridD& g, int order, const Flags4& flags, const FluxView<T>& fx, const FluxView<T>& fy,
+                   const double* cell_ez, const int* eqx, const int* eqy, double* ez, hipStream_t stream);
+template <class T>
 void launch_update(const GridD& g, double a, double b, double cw, const double* dt_dev, const StateView& un,
-                   const StateView& us, const FluxView<double>& fx, const FluxView<double>& fy, const double* ez,
+                   const StateView& us, const FluxView<T>& fx, const FluxView<T>& fy, const double* ez,
                    const StateOut& out, hipStream_t stream);
 void launch_rk_combine(const GridD& g, double a, double b, double c, const StateView& un, const StateView& us,
                        const StateView& dudt, const StateOut& out, hipStream_t stream);

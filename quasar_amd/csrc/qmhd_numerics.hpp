@@ -71,6 +71,12 @@ __host__ __device__ __forceinline__ bool fin(T x) {
   return __builtin_isfinite(x);
 }
 
+// |x| and sqrt(x) in the argument's own precision.
+__host__ __device__ __forceinline__ double qabs(double x) { return __builtin_fabs(x); }
+__host__ __device__ __forceinline__ float qabs(float x) { return __builtin_fabsf(x); }
+__host__ __device__ __forceinline__ double qsqrt(double x) { return __builtin_sqrt(x); }
+__host__ __device__ __forceinline__ float qsqrt(float x) { return __builtin_sqrtf(x); }
+
 // x/y swap into the +x-normal frame (self-inverse).
 template <class T>
 __host__ __device__ __forceinline__ St<T> rot(const St<T>& s, int dir) {
@@ -107,7 +113,7 @@ __host__ __device__ __forceinline__ T fast_speed_x(T rho, T p, T bx, T by, T bz,
   const T sum = a2 + b2;
   T disc = sum * sum - T(4) * a2 * c2;
   disc = disc < T(0) ? T(0) : disc;
-  return __builtin_sqrt(T(0.5) * (sum + __builtin_sqrt(disc)));
+  return qsqrt(T(0.5) * (sum + qsqrt(disc)));
 }
 
 // ---------------------------------------------------------------------------
@@ -115,7 +121,7 @@ __host__ __device__ __forceinline__ T fast_speed_x(T rho, T p, T bx, T by, T bz,
 template <class T>
 __host__ __device__ __forceinline__ T minmod2(T a, T b) {
   if (a == T(0) || b == T(0) || ((a < T(0)) != (b < T(0)))) return T(0);
-  return (__builtin_fabs(a) < __builtin_fabs(b)) ? a : b;
+  return (qabs(a) < qabs(b)) ? a : b;
 }
 
 template <class T>
@@ -127,8 +133,8 @@ template <class T>
 __host__ __device__ __forceinline__ T minmod4(T a, T b, T c, T d) {
   const T sa = sgn0(a), sb = sgn0(b), sc = sgn0(c), sd = sgn0(d);
   if (!(sa == sb && sb == sc && sc == sd) || sa == T(0)) return T(0);
-  return sa * fmin(fmin(__builtin_fabs(a), __builtin_fabs(b)),
-                   fmin(__builtin_fabs(c), __builtin_fabs(d)));
+  return sa * fmin(fmin(qabs(a), qabs(b)),
+                   fmin(qabs(c), qabs(d)));
 }
 
 template <class T>
@@ -139,11 +145,11 @@ __host__ __device__ __forceinline__ T mp_limit(T vl, T vm2, T vm1, T v0, T vp1, 
   const T cfc = vl - v0;
   const T cfm = vl - vmp;
   const bool between = (cfc <= T(0) && cfm >= T(0)) || (cfc >= T(0) && cfm <= T(0));
-  const T variation = fmax(fmax(__builtin_fabs(d_minus), __builtin_fabs(d_plus)),
-                           fmax(__builtin_fabs(cfc), __builtin_fabs(cfm)));
+  const T variation = fmax(fmax(qabs(d_minus), qabs(d_plus)),
+                           fmax(qabs(cfc), qabs(cfm)));
   // (|cfc|/v)(|cfm|/v) <= 1e-10, evaluated as |cfc||cfm| <= 1e-10 v^2.
   const bool within = variation > T(0) &&
-                      __builtin_fabs(cfc) * __builtin_fabs(cfm) <= T(1e-10) * (variation * variation);
+                      qabs(cfc) * qabs(cfm) <= T(1e-10) * (variation * variation);
   if (between || within) return vl;
   const T dm = (vm2 - vm1) - (vm1 - v0);
   const T d0 = (vm1 - v0) - (v0 - vp1);
@@ -189,16 +195,16 @@ __host__ __device__ __forceinline__ T mp7_face(const T* v) {
 template <class T>
 __host__ __device__ __forceinline__ T scaled_norm3(T x, T y, T z) {
 #pragma clang fp contract(off)
-  const T scale = fmax(__builtin_fabs(x), fmax(__builtin_fabs(y), __builtin_fabs(z)));
+  const T scale = fmax(qabs(x), fmax(qabs(y), qabs(z)));
   if (scale == T(0)) return T(0);
   const T xs = x / scale, ys = y / scale, zs = z / scale;
-  return scale * __builtin_sqrt(xs * xs + ys * ys + zs * zs);
+  return scale * qsqrt(xs * xs + ys * ys + zs * zs);
 }
 
 template <class T>
 __host__ __device__ __forceinline__ T half_squared_norm3(T x, T y, T z) {
 #pragma clang fp contract(off)
-  const T scale = fmax(__builtin_fabs(x), fmax(__builtin_fabs(y), __builtin_fabs(z)));
+  const T scale = fmax(qabs(x), fmax(qabs(y), qabs(z)));
   if (scale == T(0)) return T(0);
   const T xs = x / scale, ys = y / scale, zs = z / scale;
   const T ns = xs * xs + ys * ys + zs * zs;
@@ -208,10 +214,10 @@ __host__ __device__ __forceinline__ T half_squared_norm3(T x, T y, T z) {
 template <class T>
 __host__ __device__ __forceinline__ T kinetic_from_momentum(T mx, T my, T mz, T rho) {
 #pragma clang fp contract(off)
-  const T scale = fmax(__builtin_fabs(mx), fmax(__builtin_fabs(my), __builtin_fabs(mz)));
+  const T scale = fmax(qabs(mx), fmax(qabs(my), qabs(mz)));
   if (scale == T(0)) return T(0);
   const T xs = mx / scale, ys = my / scale, zs = mz / scale;
-  const T q = scale / __builtin_sqrt(rho);
+  const T q = scale / qsqrt(rho);
   const T ns = xs * xs + ys * ys + zs * zs;
   return (T(0.5) * q) * (q * ns);
 }
@@ -228,10 +234,10 @@ struct Prim {
 template <class T>
 __host__ __device__ __forceinline__ T kinetic_from_velocity(T rho, T vx, T vy, T vz) {
 #pragma clang fp contract(off)
-  const T scale = fmax(__builtin_fabs(vx), fmax(__builtin_fabs(vy), __builtin_fabs(vz)));
+  const T scale = fmax(qabs(vx), fmax(qabs(vy), qabs(vz)));
   if (scale == T(0)) return T(0);
   const T xs = vx / scale, ys = vy / scale, zs = vz / scale;
-  const T q = __builtin_sqrt(rho) * scale;
+  const T q = qsqrt(rho) * scale;
   const T ns = xs * xs + ys * ys + zs * zs;
   return (T(0.5) * q) * (q * ns);
 }
@@ -275,7 +281,7 @@ __host__ __device__ __forceinline__ bool pressure_positive(const St<T>& u, T gam
   const T mag = T(0.5) * (u.bx * u.bx + u.by * u.by + u.bz * u.bz);
   const T gm1 = gamma - T(1);
   const T pf = gm1 * ((u.en - kin) - mag);
-  const T margin = T(32) * Lim<T>::eps * gm1 * (__builtin_fabs(u.en) + kin + mag);
+  const T margin = T(32) * Lim<T>::eps * gm1 * (qabs(u.en) + kin + mag);
   if (fin(pf) && fin(margin) && pf > margin) return true;
   if (fin(pf) && fin(margin) && pf < -margin) return false;
   const T p = ref_pressure(u, gamma);
@@ -399,15 +405,15 @@ __host__ __device__ __forceinline__ void basis_build(const St<T>& s, T gamma, Ch
   const T vn = s.mx / rho, vt1 = s.my / rho, vt2 = s.mz / rho;
   T p = (gamma - T(1)) * ((s.en - kinetic_from_momentum(s.mx, s.my, s.mz, rho)) - half_squared_norm3(s.bx, s.by, s.bz));
   p = p > T(0) ? p : T(0);
-  const T sqrt_rho = __builtin_sqrt(rho);
+  const T sqrt_rho = qsqrt(rho);
   const T bn = s.bx + T(0), bt1 = s.by + T(0), bt2 = s.bz + T(0);
   // Magnetosonic speeds through the normalised quadratic (mhd_state.hpp:141-180).
   T cf, cs, a, ca;
   {
-    a = (p > T(0) && gamma > T(0)) ? __builtin_sqrt(gamma) * (__builtin_sqrt(p) / sqrt_rho) : T(0);
+    a = (p > T(0) && gamma > T(0)) ? qsqrt(gamma) * (qsqrt(p) / sqrt_rho) : T(0);
     const T ax = bn / sqrt_rho, ay = bt1 / sqrt_rho, az = bt2 / sqrt_rho;
     ca = scaled_norm3(ax, ay, az);
-    const T cax = __builtin_fabs(ax);
+    const T cax = qabs(ax);
     const T scale = fmax(a, ca);
     if (!(scale > T(0))) {
       cf = T(0);
@@ -418,14 +424,14 @@ __host__ __device__ __forceinline__ void basis_build(const St<T>& s, T gamma, Ch
       const T sum = a2n + b2n;
       T disc = sum * sum - T(4) * a2n * can2;
       if (disc < T(0)) disc = T(0);
-      const T fast2n = T(0.5) * (sum + __builtin_sqrt(disc));
+      const T fast2n = T(0.5) * (sum + qsqrt(disc));
       const T slow2n = fast2n > T(0) ? (a2n * can2) / fast2n : T(0);
-      cf = scale * __builtin_sqrt(fast2n);
-      cs = scale * __builtin_sqrt(slow2n);
+      cf = scale * qsqrt(fast2n);
+      cs = scale * qsqrt(slow2n);
     }
   }
   constexpr T geps = T(64) * Lim<T>::eps;
-  const T rs2 = T(1) / __builtin_sqrt(T(2));
+  const T rs2 = T(1) / qsqrt(T(2));
   T alpha_f, alpha_s;
   {
     const T speed_scale = cf > T(0) ? cf : T(1);
@@ -439,8 +445,8 @@ __host__ __device__ __forceinline__ void basis_build(const St<T>& s, T gamma, Ch
       T as2 = (cfn * cfn - an * an) / denom;
       if (af2 < T(0)) af2 = T(0);
       if (as2 < T(0)) as2 = T(0);
-      alpha_f = __builtin_sqrt(af2);
-      alpha_s = __builtin_sqrt(as2);
+      alpha_f = qsqrt(af2);
+      alpha_s = qsqrt(as2);
     }
   }
   T beta1, beta2;
@@ -510,7 +516,7 @@ __host__ __device__ __forceinline__ void basis_build(const St<T>& s, T gamma, Ch
 #pragma unroll
       for (int k = 1; k < 7; ++k) dot = dot + row[k] * R[k][j];
       const T expect = (i == j) ? T(1) : T(0);
-      usable = usable && fin(dot) && !(__builtin_fabs(dot - expect) > T(1024) * Lim<T>::eps);
+      usable = usable && fin(dot) && !(qabs(dot - expect) > T(1024) * Lim<T>::eps);
     }
   }
   B.identity = !usable;
@@ -619,7 +625,7 @@ __host__ __device__ __forceinline__ St<T> hll_flux(const St<T>& L, const St<T>& 
   const St<T> fr = phys_flux(R, gm1);
   if (sl >= T(0)) return fl;
   if (sr <= T(0)) return fr;
-  const T scale = fmax(__builtin_fabs(sl), __builtin_fabs(sr));
+  const T scale = fmax(qabs(sl), qabs(sr));
   if (!(scale > T(0)) || !fin(scale)) return fl;
   const T denom = sr - sl;
   if (!(denom > T(0)) || !fin(denom)) return fl;
@@ -642,15 +648,15 @@ __host__ __device__ __forceinline__ St<T> lf_flux(const St<T>& L, const St<T>& R
   const T gm1 = gamma - T(1);
   const T cfL = fast_speed_x(L.rho, pressure(L, gm1), L.bx, L.by, L.bz, gamma);
   const T cfR = fast_speed_x(R.rho, pressure(R, gm1), R.bx, R.by, R.bz, gamma);
-  const T aL = __builtin_fabs(L.mx / L.rho) + cfL;
-  const T aR = __builtin_fabs(R.mx / R.rho) + cfR;
+  const T aL = qabs(L.mx / L.rho) + cfL;
+  const T aR = qabs(R.mx / R.rho) + cfR;
   const T a = aL > aR ? aL : aR;
   return hll_flux(L, R, -a, a, gm1);
 }
 
 template <class T>
 __host__ __device__ __forceinline__ bool resolved(T value, T scale) {
-  return fin(value) && __builtin_fabs(value) > T(64) * Lim<T>::eps * (scale + Lim<T>::tiny);
+  return fin(value) && qabs(value) > T(64) * Lim<T>::eps * (scale + Lim<T>::tiny);
 }
 
 template <class T>
@@ -684,14 +690,14 @@ __host__ __device__ __forceinline__ St<T> hlld_flux(const St<T>& Lin, const St<T
   const T aR = (sr - vxR) * R.rho;
   const T aL = (sl - vxL) * L.rho;
   const T denomM = aR - aL;
-  if (!resolved(denomM, __builtin_fabs(aR) + __builtin_fabs(aL))) return hll_flux(L, R, sl, sr, gm1);
+  if (!resolved(denomM, qabs(aR) + qabs(aL))) return hll_flux(L, R, sl, sr, gm1);
   const T inv_dM = T(1) / denomM;
   const T sm = (aR * vxR - aL * vxL - ptR + ptL) * inv_dM;
   if (!fin(sm)) return hll_flux(L, R, sl, sr, gm1);
   const T pts = (aR * ptL - aL * ptR + aL * aR * (vxR - vxL)) * inv_dM;
   if (!fin(pts)) return hll_flux(L, R, sl, sr, gm1);
   const T dLs = sl - sm, dRs = sr - sm;
-  const T wave_scale = __builtin_fabs(sl) + __builtin_fabs(sr) + __builtin_fabs(sm);
+  const T wave_scale = qabs(sl) + qabs(sr) + qabs(sm);
   if (!resolved(dLs, wave_scale) || !resolved(dRs, wave_scale)) return hll_flux(L, R, sl, sr, gm1);
   const T rhoLs = aL / dLs;
   const T rhoRs = aR / dRs;
@@ -703,7 +709,7 @@ __host__ __device__ __forceinline__ St<T> hlld_flux(const St<T>& Lin, const St<T
     const T smv = sl - vxL;
     const T q = L.rho * smv * dLs;
     const T den = q - bn2;
-    if (!resolved(den, __builtin_fabs(q) + bn2)) return hll_flux(L, R, sl, sr, gm1);
+    if (!resolved(den, qabs(q) + bn2)) return hll_flux(L, R, sl, sr, gm1);
     const T id = T(1) / den;
     const T fac = bn * (sm - vxL) * id;
     vyLs = vyL - L.by * fac;
@@ -719,7 +725,7 @@ __host__ __device__ __forceinline__ St<T> hlld_flux(const St<T>& Lin, const St<T
     const T smv = sr - vxR;
     const T q = R.rho * smv * dRs;
     const T den = q - bn2;
-    if (!resolved(den, __builtin_fabs(q) + bn2)) return hll_flux(L, R, sl, sr, gm1);
+    if (!resolved(den, qabs(q) + bn2)) return hll_flux(L, R, sl, sr, gm1);
     const T id = T(1) / den;
     const T fac = bn * (sm - vxR) * id;
     vyRs = vyR - R.by * fac;
@@ -733,12 +739,12 @@ __host__ __device__ __forceinline__ St<T> hlld_flux(const St<T>& Lin, const St<T
   }
   const St<T> usL{rhoLs, rhoLs * sm, rhoLs * vyLs, rhoLs * vzLs, eLs, bn, byLs, bzLs};
   const St<T> usR{rhoRs, rhoRs * sm, rhoRs * vyRs, rhoRs * vzRs, eRs, bn, byRs, bzRs};
-  const T sqL = __builtin_sqrt(rhoLs), sqR = __builtin_sqrt(rhoRs);
-  const T abn = __builtin_fabs(bn);
+  const T sqL = qsqrt(rhoLs), sqR = qsqrt(rhoRs);
+  const T abn = qabs(bn);
   const T saL = sm - abn / sqL;
   const T saR = sm + abn / sqR;
   const bool bn_negligible =
-      bn2 <= T(64) * Lim<T>::eps * (__builtin_fabs(pts) + __builtin_fabs(ptL) + __builtin_fabs(ptR) + Lim<T>::tiny);
+      bn2 <= T(64) * Lim<T>::eps * (qabs(pts) + qabs(ptL) + qabs(ptR) + Lim<T>::tiny);
   St<T> f;
   if (bn_negligible) {
     const bool left = sl <= T(0) && T(0) <= sm;

@@ -180,6 +180,20 @@ FluxView<double> fluxv(double* const* p) {
   for (int c = 0; c < 8; ++c) v.c[c] = p[c];
   return v;
 }
+// Views of the solver's face planes in the arithmetic type of the face work
+// (FP32 planes occupy the first half of each FP64-sized buffer).
+template <class T>
+IfaceView<T> iface_as(double* const* p) {
+  IfaceView<T> v;
+  for (int c = 0; c < 16; ++c) v.c[c] = reinterpret_cast<T*>(p[c]);
+  return v;
+}
+template <class T>
+FluxView<T> flux_as(double* const* p) {
+  FluxView<T> v;
+  for (int c = 0; c < 8; ++c) v.c[c] = reinterpret_cast<T*>(p[c]);
+  return v;
+}
 
 double bits_to_double(unsigned long long b) {
   double d;
@@ -288,6 +302,11 @@ class Solver {
     if (geometry != "cartesian")
       throw std::invalid_argument("MhdSolver2D: this build implements geometry 'cartesian'");
     order_ = order_of(cfg.reconstruction);
+    const std::string arithmetic = cfg.arithmetic[0] ? cfg.arithmetic : "fp64";
+    if (arithmetic == "fp32")
+      fp32_ = true;
+    else if (arithmetic != "fp64")
+      throw std::invalid_argument("MhdSolver2D: arithmetic must be 'fp64' or 'fp32'");
     if (std::string(cfg.riemann) != "hlld")
       throw std::invalid_argument("MhdSolver2D: device evolution supports only riemann='hlld'");
     if (std::string(cfg.ct) != "fd_ct_christlieb")
@@ -450,19 +469,44 @@ class Solver {
       fyp[c] = fy_[c].p;
     }
     (void)want_rates;
+    if (fp32_)
+      issue_faces_as<float>(r, order, hll, ip, fxp, fyp);
+    else
+      issue_faces_as<double>(r, order, hll, ip, fxp, fyp);
+  }
+
+  template <class T>
+  void issue_faces_as(Register& r, int order, int hll, double* const* ip, double* const* fxp, double* const* fyp) {
     clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_, [&] {
-      launch_reconstruct(g_, 0, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface(ip),
-                         reinterpret_cast<int*>(eqx_.p), fb_count(), fb_list(), stream_);
+      launch_reconstruct<T>(g_, 0, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface_as<T>(ip),
+                            reinterpret_cast<int*>(eqx_.p), fb_count(), fb_list(), stream_);
     });
     clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
-      launch_flux(g_, 0, order, hll, field_, gamma_, iface(ip), fluxv(fxp), rate_x_.p, stream_);
+      launch_flux<T>(g_, 0, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fxp), rate_x_.p, stream_);
     });
     clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_, [&] {
-      launch_reconstruct(g_, 1, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface(ip),
-                         reinterpret_cast<int*>(eqy_.p), fb_count(), fb_list(), stream_);
+      launch_reconstruct<T>(g_, 1, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface_as<T>(ip),
+                            reinterpret_cast<int*>(eqy_.p), fb_count(), fb_list(), stream_);
     });
     clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
-      launch_flux(g_, 1, order, hll, field_, gamma_, iface(ip), fluxv(fyp), rate_y_.p, stream_);
+      launch_flux<T>(g_, 1, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fyp), rate_y_.p, stream_);
+    });
+  }
+
+  // Cell E_z, corner E_z and the fused update of one stage.
+  template <class T>
+  void issue_ct_update(Register& in, Register& out, double a, double b, double c_host, const double* dt_dev,
+                       const Register& us, int order, double* const* fxp, double* const* fyp) {
+    clock_.timed(QMHD_KERNEL_CELL_EMF, stream_, [&] {
+      launch_cell_emf<T>(g_, order, field_, gamma_, in.view(), in.bxc.p, in.byc.p, cell_ez_.p, stream_);
+    });
+    clock_.timed(QMHD_KERNEL_CORNER, stream_, [&] {
+      launch_corner<T>(g_, order, field_, flux_as<T>(fxp), flux_as<T>(fyp), cell_ez_.p,
+                       reinterpret_cast<int*>(eqx_.p), reinterpret_cast<int*>(eqy_.p), ez_.p, stream_);
+    });
+    clock_.timed(QMHD_KERNEL_UPDATE, stream_, [&] {
+      launch_update<T>(g_, a, b, c_host, dt_dev, reg_[L_].view(), us.view(), flux_as<T>(fxp), flux_as<T>(fyp),
+                       ez_.p, out.out(), stream_);
     });
   }
 
@@ -504,18 +548,12 @@ class Solver {
       Register& in = reg_[in_slot[s]];
       Register& out = reg_[out_slot[s]];
       if (s > 0) issue_faces(in_slot[s], order, false);
-      clock_.timed(QMHD_KERNEL_CELL_EMF, stream_, [&] {
-        launch_cell_emf(g_, order, field_, gamma_, in.view(), in.bxc.p, in.byc.p, cell_ez_.p, stream_);
-      });
-      clock_.timed(QMHD_KERNEL_CORNER, stream_, [&] {
-        launch_corner(g_, order, field_, fluxv(fxp), fluxv(fyp), cell_ez_.p, reinterpret_cast<int*>(eqx_.p),
-                      reinterpret_cast<int*>(eqy_.p), ez_.p, stream_);
-      });
       const double c_host = dt_dev ? cw[s] : (s == 0 ? dt : cw[s] * dt);
-      clock_.timed(QMHD_KERNEL_UPDATE, stream_, [&] {
-        launch_update(g_, a[s], b[s], c_host, dt_dev, reg_[L_].view(), reg_[s == 2 ? S2_ : S1_].view(),
-                      fluxv(fxp), fluxv(fyp), ez_.p, out.out(), stream_);
-      });
+      const Register& us = reg_[s == 2 ? S2_ : S1_];
+      if (fp32_)
+        issue_ct_update<float>(in, out, a[s], b[s], c_host, dt_dev, us, order, fxp, fyp);
+      else
+        issue_ct_update<double>(in, out, a[s], b[s], c_host, dt_dev, us, order, fxp, fyp);
       clock_.timed(QMHD_KERNEL_GHOSTS, stream_, [&] { launch_ghosts(g_, fluid_, field_, out.out(), stream_); });
       clock_.timed(QMHD_KERNEL_PREP, stream_, [&] {
         launch_prep(g_, out.view(), out.bxc.p, out.byc.p, reg_[L_].view(), reg_[L_].bxc.p, reg_[L_].byc.p, s,
@@ -698,6 +736,7 @@ class Solver {
   qmhd_grid grid_pub_{};
   double gamma_{5.0 / 3.0}, cfl_{0.4};
   int order_{7};
+  bool fp32_{false};  // FP32 face arithmetic (reconstruction, fluxes, cell E_z quadrature)
   Flags4 fluid_{}, field_{};
   hipStream_t stream_{nullptr};
   Register reg_[4];
@@ -938,7 +977,7 @@ int qmhd_ct_emf(const qmhd_grid* g, int order, const int32_t* flags4, double gam
     int* eqy = reinterpret_cast<int*>(s.plane(4));
     qmhd::launch_eq_flags(d, 0, qmhd::iface(const_cast<double* const*>(ifx16)), eqx, st(stream));
     qmhd::launch_eq_flags(d, 1, qmhd::iface(const_cast<double* const*>(ify16)), eqy, st(stream));
-    qmhd::launch_cell_emf(d, order, f, gamma, qmhd::view(u8), s.plane(0), s.plane(1), s.plane(2), st(stream));
+    qmhd::launch_cell_emf<double>(d, order, f, gamma, qmhd::view(u8), s.plane(0), s.plane(1), s.plane(2), st(stream));
     qmhd::launch_corner(d, order, f, qmhd::fluxv(const_cast<double* const*>(flux_x8)),
                         qmhd::fluxv(const_cast<double* const*>(flux_y8)), s.plane(2), eqx, eqy, ez_edge,
                         st(stream));

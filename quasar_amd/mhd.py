@@ -186,12 +186,14 @@ class MhdConfig:
         self.timestep_signature = ""
         self.boundary = MhdBoundarySpec()
         self.background = MhdBackgroundSpec()
+        # Extension: "fp64" (the reference's arithmetic) or "fp32" face work.
+        self.arithmetic = "fp64"
 
     def _c(self) -> _lib.QSolverConfig:
         c = _lib.QSolverConfig()
         c.grid = self.grid._c()
         c.gamma, c.cfl, c.rho_floor, c.p_floor = self.gamma, self.cfl, self.rho_floor, self.p_floor
-        for name in ("reconstruction", "riemann", "integrator", "ct", "positivity", "geometry"):
+        for name in ("reconstruction", "riemann", "integrator", "ct", "positivity", "geometry", "arithmetic"):
             setattr(c, name, getattr(self, name).encode())
         for s in range(4):
             c.fluid[s].value = self.boundary.fluid[s].encode()

@@ -37,3 +37,20 @@ def reference():
         pytest.skip("reference numerics library oracle/_ref was not built "
                     "(reference tree absent)")
     return Reference()
+
+
+_torch_device_ready = False
+
+
+@pytest.fixture(autouse=True)
+def _torch_device_first(request):
+    """GPU tests move planes through torch tensors.  torch's HIP runtime is
+    initialised before the library's own (which a solver created first would
+    do), once per session."""
+    global _torch_device_ready
+    if request.node.get_closest_marker("gpu") is not None and not _torch_device_ready:
+        import torch
+        assert torch.cuda.is_available(), "GPU test without a visible HIP device"
+        torch.cuda.init()
+        _torch_device_ready = True
+    yield
