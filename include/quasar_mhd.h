@@ -161,6 +161,10 @@ typedef struct {
    * FP32; the state, collocation, positivity recompute, CFL rates and
    * reduction, corner E_z, update and conserved sums stay FP64. */
   char arithmetic[32];
+  /* Extension (no reference counterpart): troubled-cell indicator, "" or
+   * "none" (the plain method) or "mlp" (the learned form, MP5/MP7 only; the
+   * network is set with qmhd_solver_set_indicator before the first step). */
+  char indicator[32];
 } qmhd_solver_config;
 
 int qmhd_solver_create(const qmhd_solver_config* cfg, qmhd_solver** out);
@@ -202,7 +206,8 @@ enum {
   QMHD_KERNEL_UPDATE = 5,
   QMHD_KERNEL_GHOSTS = 6,
   QMHD_KERNEL_REDUCE = 7,
-  QMHD_KERNEL_KINDS = 8
+  QMHD_KERNEL_INDICATOR = 8,
+  QMHD_KERNEL_KINDS = 9
 };
 int qmhd_solver_set_profiling(qmhd_solver* s, int enable);
 int qmhd_solver_kernel_time(qmhd_solver* s, int kind, double* total_ms, long long* launches);
@@ -212,6 +217,67 @@ void* qmhd_solver_stream(qmhd_solver* s);
 /* Wait for all work issued by the solver (device_synchronize in the
  * reference's compute_residual, src/physics/mhd/mhd_solver.cpp:825). */
 int qmhd_solver_synchronize(qmhd_solver* s);
+
+/* ---- learned form (no reference counterpart) -------------------------
+ * A per-cell troubled-cell indicator: 100 features (the 5x5 stencil of
+ * rho, p, |B|, |v|, each as (Q - Q_centre) / (|Q_centre| + max |Q - Q_centre|))
+ * into a 100-32-32-1 ReLU network evaluated with BF16 MFMA and FP32
+ * accumulation.  A face whose two cells are unflagged takes the unlimited
+ * linear fifth-order face values and the HLL flux; every other face takes the
+ * plain MP + HLLD path.  Parameters are one FP32 vector: W1[100][32], b1[32],
+ * W2[32][32], b2[32], w3[32], b3. */
+#define QMHD_MLP_FEATURES 100
+#define QMHD_MLP_HIDDEN 32
+#define QMHD_MLP_PARAMS 4321
+#define QMHD_MLP_PACKED 5632 /* uint16_t BF16 operands */
+/* Label bits of a face (cells: OR of their four faces). */
+/* label_tol: relative size above which a limiter change (against the largest
+ * component of the face's reference state) or an HLLD-HLL flux gap (against
+ * the largest HLLD flux component) counts. */
+#define QMHD_LABEL_MP_CLIP 1     /* an MP limiter call changed a face value by more than label_tol */
+#define QMHD_LABEL_MUSCL 2       /* MUSCL fallback */
+#define QMHD_LABEL_CONSTANT 4    /* piecewise-constant fallback or NaN sentinel */
+#define QMHD_LABEL_HLL_GAP 8     /* max|F_HLLD - F_HLL| > label_tol * max|F_HLLD| */
+
+/* BF16 operand layout of the network (run after every parameter change). */
+int qmhd_mlp_pack(const float* params, uint16_t* packed, void* stream);
+/* Logits of n samples (features: n x 100 FP32, row-major). */
+int qmhd_mlp_forward(const uint16_t* packed, const float* params, const float* features, int64_t n, float* logits,
+                     void* stream);
+/* Floats of the `partial` workspace of qmhd_mlp_gradient. */
+size_t qmhd_mlp_partial_floats(int64_t n, int chunks);
+/* Gradient of sum_s w_s * BCE(logit_s, y_s) over the samples order[0..n)
+ * (identity when order is NULL) into grad_and_loss[0..QMHD_MLP_PARAMS), the
+ * loss into grad_and_loss[QMHD_MLP_PARAMS]; deterministic; `chunks` passes of
+ * 16 samples per wave. */
+int qmhd_mlp_gradient(const uint16_t* packed, const float* params, const float* features, const float* targets,
+                      const float* weights, const int64_t* order, int64_t n, int chunks, float* partial,
+                      float* grad_and_loss, void* stream);
+/* One Adam step (bias-corrected, `step` counts from 1). */
+int qmhd_mlp_adam(float* params, const float* grad, float* m, float* v, int64_t step, float lr, float beta1,
+                  float beta2, float eps, void* stream);
+/* Features of every interior cell of a ghost-filled state (nx*ny x 100 FP32,
+ * cell (i,j) at row j*nx+i); scratch as qmhd_scratch_doubles. */
+int qmhd_cell_features(const qmhd_grid* g, double gamma, const double* const* u8, float* features, double* scratch,
+                       void* stream);
+/* Label bits of every interior cell (nx*ny bytes) from the plain FP64 MP5/MP7
+ * method on a ghost-filled state. */
+int qmhd_cell_labels(const qmhd_grid* g, int order, const int32_t* flags4, double gamma, const double* const* u8,
+                     double label_tol, uint8_t* cell_bits, double* scratch, void* stream);
+/* The solver's indicator on a ghost-filled state: flags (1 = plain path) of
+ * every padded cell (storage-size bytes). */
+int qmhd_cell_indicator(const qmhd_grid* g, double gamma, const double* const* u8, const uint16_t* packed,
+                        const float* params, float threshold_logit, uint8_t* flags, double* scratch, void* stream);
+/* Wait for the work queued on `stream` (NULL: the library's null stream). */
+int qmhd_stream_synchronize(void* stream);
+/* Network of a solver configured with indicator "mlp": cells whose logit is
+ * below threshold_logit take the cheap path. */
+int qmhd_solver_set_indicator(qmhd_solver* s, const float* host_params, float threshold_logit);
+/* Faces sent to the plain path and faces reconstructed since the last call. */
+int qmhd_solver_plain_faces(qmhd_solver* s, int64_t* plain, int64_t* total);
+/* Features / label bits of the live state (device outputs, as above). */
+int qmhd_solver_cell_features(qmhd_solver* s, float* features);
+int qmhd_solver_cell_labels(qmhd_solver* s, double label_tol, uint8_t* cell_bits);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,19 @@ int orc_solver_last_substeps(void* h);
 void orc_solver_counters(void* h, OrcCounters* out);
 const char* orc_last_error(void);
 
+/* Learned form (no reference counterpart): the troubled-cell features, the
+ * labels from the plain method's own MP limiter, fallback chain and HLLD-vs-HLL
+ * gap, and the 100-32-32-1 network in FP64.  bf16 != 0 rounds the operands
+ * of the device's BF16 products (inputs, W1, W2, hidden activations and the
+ * backward deltas) as the device does; bf16 == 0 is the exact network. */
+void orc_cell_features(const OrcGrid* g, double gamma, double* const* u8, float* out);
+void orc_cell_labels(const OrcGrid* g, int order, const int* flags, double gamma, double* const* u8,
+                     double label_tol, uint8_t* cell_bits);
+void orc_hll_davis_flux_x(const double* l, const double* r, double gamma, double* f);
+void orc_mlp_forward(const float* params, const float* x, long long n, int bf16, double* logits);
+double orc_mlp_loss_grad(const float* params, const double* params64, const float* x, const float* y,
+                         const float* w, long long n, int bf16, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

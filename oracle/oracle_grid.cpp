@@ -708,10 +708,109 @@ State load_cell_state(const Grid& g, const Field& u, int i, int j, int co) {  //
   return s;
 }
 
-void reconstruct(const Grid& g, int dir, int order, const Flags& flags, Real gamma,
-                 const Field& u, Iface& out, Counters* counters) {  // mhd_reconstruct.hip:465-670
+// One face of reconstruct(); `bits` (optional) receives the label path bits
+// (1 an MP limiter changed a face value by more than label_tol of the largest
+// reference-state component, 2 MUSCL fallback, 4 piecewise-constant fallback
+// or NaN sentinel).
+void reconstruct_one_face(const Grid& g, int dir, int i, int j, int order, int side_lo, int side_hi, Real gamma,
+                          const Field& u, State& ul, State& ur, Counters* counters, unsigned* bits,
+                          Real label_tol) {
   const int di = (dir == 0) ? 1 : 0;
   const int dj = (dir == 0) ? 0 : 1;
+  if (order <= 1) {
+    ul = load_cell_state(g, u, i - di, j - dj, 1);
+    ur = load_cell_state(g, u, i, j, 1);
+  } else if (order != 2) {
+    const State cL = load_cell_state(g, u, i - di, j - dj, 0);
+    const State cR = load_cell_state(g, u, i, j, 0);
+    State ref;
+    for (int c = 0; c < 8; ++c)
+      set_comp(ref, c, Real{0.5} * comp(cL, c) + Real{0.5} * comp(cR, c));
+    const State cref = rotate(ref, dir);
+    Eigen eig;
+    eigen_build(cref, gamma, eig);
+    const int half = (order == 5) ? 2 : 3;
+    const int uw = 2 * half + 2;
+    Real chars[8][7];
+    for (int uu = 0; uu < uw; ++uu) {
+      const int off = -1 - half + uu;
+      const State loaded = load_cell_state(g, u, i + off * di, j + off * dj, 0);
+      State delta = rotate(loaded, dir);
+      for (int c = 0; c < 8; ++c) set_comp(delta, c, comp(delta, c) - comp(cref, c));
+      to_char(delta, eig, chars[uu]);
+    }
+    const int width = 2 * half + 1;
+    State faces[2];
+    Real shift = 0;
+    for (int side = 0; side < 2; ++side) {
+      const bool right = side == 1;
+      const int base = right ? 1 : 0;
+      Real wf[7];
+      for (int k = 0; k < 7; ++k) {
+        Real scalar[7];
+        for (int s = 0; s < width; ++s) {
+          const int src = right ? (width - 1 - s) : s;
+          scalar[s] = chars[base + src][k];
+        }
+        Real moved = 0;
+        wf[k] = half == 2 ? mp5_reconstruct(scalar, &moved) : mp7_reconstruct(scalar, &moved);
+        shift = std::fmax(shift, moved);
+        if (counters && moved > 0) {
+#pragma omp atomic
+          counters->mp_clips++;
+        }
+      }
+      const State d = from_char(wf, eig);
+      State s;
+      for (int c = 0; c < 8; ++c) set_comp(s, c, comp(cref, c) + comp(d, c));
+      faces[side] = rotate(s, dir);
+    }
+    ul = faces[0];
+    ur = faces[1];
+    if (bits) {
+      Real scale = 0;
+      for (int c = 0; c < 8; ++c) scale = std::fmax(scale, std::fabs(comp(cref, c)));
+      if (shift / (scale + std::numeric_limits<Real>::min()) > label_tol) *bits |= 1u;
+    }
+    if (!reconstructed_state_admissible(ul, gamma) || !reconstructed_state_admissible(ur, gamma)) {
+      if (bits) *bits |= 2u;
+      if (counters) {
+#pragma omp atomic
+        counters->muscl_fallbacks++;
+      }
+      muscl_reconstruct_face(g, dir, i, j, gamma, side_lo, side_hi, u, ul, ur);
+    }
+  } else {
+    muscl_reconstruct_face(g, dir, i, j, gamma, side_lo, side_hi, u, ul, ur);
+  }
+  const Real bn = (dir == 0 ? u.c[5] : u.c[6])[g.index(i, j)];
+  set_normal_b(ul, dir, bn);
+  set_normal_b(ur, dir, bn);
+  if (!reconstructed_state_admissible(ul, gamma) || !reconstructed_state_admissible(ur, gamma)) {
+    if (bits) *bits |= 4u;
+    if (counters) {
+#pragma omp atomic
+      counters->pc_fallbacks++;
+    }
+    const int fco = (order <= 1) ? 1 : 0;
+    ul = load_cell_state(g, u, i - di, j - dj, fco);
+    ur = load_cell_state(g, u, i, j, fco);
+    set_normal_b(ul, dir, bn);
+    set_normal_b(ur, dir, bn);
+  }
+  if (!reconstructed_state_admissible(ul, gamma) || !reconstructed_state_admissible(ur, gamma)) {
+    if (counters) {
+#pragma omp atomic
+      counters->sentinels++;
+    }
+    const Real nan = std::numeric_limits<Real>::quiet_NaN();
+    ul = State{nan, nan, nan, nan, nan, nan, nan, nan};
+    ur = ul;
+  }
+}
+
+void reconstruct(const Grid& g, int dir, int order, const Flags& flags, Real gamma,
+                 const Field& u, Iface& out, Counters* counters) {  // mhd_reconstruct.hip:465-670
   const int side_lo = (dir == 0) ? flags.side[0] : flags.side[2];
   const int side_hi = (dir == 0) ? flags.side[1] : flags.side[3];
   const int i_lo = dir == 0 ? 0 : -g.ng;
@@ -722,87 +821,7 @@ void reconstruct(const Grid& g, int dir, int order, const Flags& flags, Real gam
   for (int j = j_lo; j <= j_hi; ++j) {
     for (int i = i_lo; i <= i_hi; ++i) {
       State ul, ur;
-      if (order <= 1) {
-        ul = load_cell_state(g, u, i - di, j - dj, 1);
-        ur = load_cell_state(g, u, i, j, 1);
-      } else if (order != 2) {
-        const State cL = load_cell_state(g, u, i - di, j - dj, 0);
-        const State cR = load_cell_state(g, u, i, j, 0);
-        State ref;
-        for (int c = 0; c < 8; ++c)
-          set_comp(ref, c, Real{0.5} * comp(cL, c) + Real{0.5} * comp(cR, c));
-        const State cref = rotate(ref, dir);
-        Eigen eig;
-        eigen_build(cref, gamma, eig);
-        const int half = (order == 5) ? 2 : 3;
-        const int uw = 2 * half + 2;
-        Real chars[8][7];
-        for (int uu = 0; uu < uw; ++uu) {
-          const int off = -1 - half + uu;
-          const State loaded = load_cell_state(g, u, i + off * di, j + off * dj, 0);
-          State delta = rotate(loaded, dir);
-          for (int c = 0; c < 8; ++c) set_comp(delta, c, comp(delta, c) - comp(cref, c));
-          to_char(delta, eig, chars[uu]);
-        }
-        const int width = 2 * half + 1;
-        State faces[2];
-        for (int side = 0; side < 2; ++side) {
-          const bool right = side == 1;
-          const int base = right ? 1 : 0;
-          Real wf[7];
-          for (int k = 0; k < 7; ++k) {
-            Real scalar[7];
-            for (int s = 0; s < width; ++s) {
-              const int src = right ? (width - 1 - s) : s;
-              scalar[s] = chars[base + src][k];
-            }
-            int clipped = 0;
-            wf[k] = half == 2 ? mp5_reconstruct(scalar, &clipped) : mp7_reconstruct(scalar, &clipped);
-            if (counters && clipped) {
-#pragma omp atomic
-      counters->mp_clips++;
-      }
-          }
-          const State d = from_char(wf, eig);
-          State s;
-          for (int c = 0; c < 8; ++c) set_comp(s, c, comp(cref, c) + comp(d, c));
-          faces[side] = rotate(s, dir);
-        }
-        ul = faces[0];
-        ur = faces[1];
-        if (!reconstructed_state_admissible(ul, gamma) || !reconstructed_state_admissible(ur, gamma)) {
-          if (counters) {
-#pragma omp atomic
-      counters->muscl_fallbacks++;
-      }
-          muscl_reconstruct_face(g, dir, i, j, gamma, side_lo, side_hi, u, ul, ur);
-        }
-      } else {
-        muscl_reconstruct_face(g, dir, i, j, gamma, side_lo, side_hi, u, ul, ur);
-      }
-      const Real bn = (dir == 0 ? u.c[5] : u.c[6])[g.index(i, j)];
-      set_normal_b(ul, dir, bn);
-      set_normal_b(ur, dir, bn);
-      if (!reconstructed_state_admissible(ul, gamma) || !reconstructed_state_admissible(ur, gamma)) {
-        if (counters) {
-#pragma omp atomic
-      counters->pc_fallbacks++;
-      }
-        const int fco = (order <= 1) ? 1 : 0;
-        ul = load_cell_state(g, u, i - di, j - dj, fco);
-        ur = load_cell_state(g, u, i, j, fco);
-        set_normal_b(ul, dir, bn);
-        set_normal_b(ur, dir, bn);
-      }
-      if (!reconstructed_state_admissible(ul, gamma) || !reconstructed_state_admissible(ur, gamma)) {
-        if (counters) {
-#pragma omp atomic
-      counters->sentinels++;
-      }
-        const Real nan = std::numeric_limits<Real>::quiet_NaN();
-        ul = State{nan, nan, nan, nan, nan, nan, nan, nan};
-        ur = ul;
-      }
+      reconstruct_one_face(g, dir, i, j, order, side_lo, side_hi, gamma, u, ul, ur, counters, nullptr, 0.0);
       const std::size_t k = g.index(i, j);
       for (int c = 0; c < 8; ++c) {
         out.c[c][k] = comp(ul, c);

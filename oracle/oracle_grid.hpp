@@ -85,6 +85,9 @@ Real transverse_point_value(int order, int node, const Real* samples);
 Real transverse_gauss_average(int order, const Real* samples);
 
 // Kernels.
+void reconstruct_one_face(const Grid& g, int dir, int i, int j, int order, int side_lo, int side_hi, Real gamma,
+                          const Field& u, State& ul, State& ur, Counters* counters, unsigned* bits,
+                          Real label_tol);
 void reconstruct(const Grid& g, int dir, int order, const Flags& flags, Real gamma,
                  const Field& u, Iface& out, Counters* counters);
 void face_flux(const Grid& g, int dir, int order, int hll_only, const Flags& flags,

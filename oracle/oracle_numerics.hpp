@@ -333,9 +333,9 @@ inline Real minmod4(Real a, Real b, Real c, Real d) {  // :46-56
 
 inline Real median3(Real a, Real b, Real c) { return a + minmod2(b - a, c - a); }
 
-// Flag bits reported by mp_limit for label generation: 1 = clipped.
+// Label generation: *shift_max receives the largest |limited - unlimited|.
 inline Real mp_limit(Real vl, Real vm2, Real vm1, Real v0, Real vp1, Real vp2,
-                     int* clipped = nullptr) {  // :66-148
+                     Real* shift_max = nullptr) {  // :66-148
   constexpr Real alpha = Real{4};
   constexpr Real eps = static_cast<Real>(1e-10);
   const Real d_minus = v0 - vm1;
@@ -350,7 +350,6 @@ inline Real mp_limit(Real vl, Real vm2, Real vm1, Real v0, Real vp1, Real vp2,
   const bool within = variation > Real{0} &&
                       (std::fabs(cfc) / variation) * (std::fabs(cfm) / variation) <= eps;
   if (between || within) return vl;
-  if (clipped) *clipped = 1;
   const Real dm = (vm2 - vm1) - (vm1 - v0);
   const Real d0 = (vm1 - v0) - (v0 - vp1);
   const Real dp = (v0 - vp1) - (vp1 - vp2);
@@ -369,7 +368,9 @@ inline Real mp_limit(Real vl, Real vm2, Real vm1, Real v0, Real vp1, Real vp2,
                               std::fmin(std::fmin(v0, v_ul), v_lc));
   const Real vmax = std::fmin(std::fmax(std::fmax(v0, vp1), v_md),
                               std::fmax(std::fmax(v0, v_ul), v_lc));
-  return median3(vl, vmin, vmax);
+  const Real limited = median3(vl, vmin, vmax);
+  if (shift_max) *shift_max = std::fmax(*shift_max, std::fabs(limited - vl));
+  return limited;
 }
 
 inline Real mp5_interp(Real a, Real b, Real c, Real d, Real e) {  // :182-191
@@ -394,13 +395,13 @@ inline Real mp7_interp(Real a, Real b, Real c, Real d, Real e, Real f, Real g) {
   return (sum / Real{420}) * scale;
 }
 
-inline Real mp5_reconstruct(const Real* v, int* clipped = nullptr) {  // :211-215
+inline Real mp5_reconstruct(const Real* v, Real* shift_max = nullptr) {  // :211-215
   return mp_limit(mp5_interp(v[0], v[1], v[2], v[3], v[4]), v[0], v[1], v[2], v[3],
-                  v[4], clipped);
+                  v[4], shift_max);
 }
-inline Real mp7_reconstruct(const Real* v, int* clipped = nullptr) {  // :219-224
+inline Real mp7_reconstruct(const Real* v, Real* shift_max = nullptr) {  // :219-224
   return mp_limit(mp7_interp(v[0], v[1], v[2], v[3], v[4], v[5], v[6]), v[1], v[2],
-                  v[3], v[4], v[5], clipped);
+                  v[3], v[4], v[5], shift_max);
 }
 
 // ---------------------------------------------------------------------------

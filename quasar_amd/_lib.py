@@ -41,7 +41,8 @@ class QSolverConfig(C.Structure):
     _fields_ = [("grid", QGrid), ("gamma", D), ("cfl", D), ("rho_floor", D), ("p_floor", D),
                 ("reconstruction", Name32), ("riemann", Name32), ("integrator", Name32),
                 ("ct", Name32), ("positivity", Name32), ("geometry", Name32),
-                ("fluid", Name32 * 4), ("field", Name32 * 4), ("arithmetic", Name32)]
+                ("fluid", Name32 * 4), ("field", Name32 * 4), ("arithmetic", Name32),
+                ("indicator", Name32)]
 
 
 EXPORTS = [
@@ -57,9 +58,19 @@ EXPORTS = [
     "qmhd_solver_run", "qmhd_solver_state_plane", "qmhd_solver_set_profiling",
     "qmhd_solver_kernel_time", "qmhd_kernel_kind_name", "qmhd_solver_stream",
     "qmhd_solver_synchronize",
+    "qmhd_mlp_pack", "qmhd_mlp_forward", "qmhd_mlp_partial_floats", "qmhd_mlp_gradient", "qmhd_mlp_adam",
+    "qmhd_cell_features", "qmhd_cell_labels", "qmhd_cell_indicator", "qmhd_stream_synchronize",
+    "qmhd_solver_set_indicator", "qmhd_solver_plain_faces", "qmhd_solver_cell_features",
+    "qmhd_solver_cell_labels",
 ]
 
-KERNEL_KINDS = ["prep", "reconstruct", "flux", "cell_emf", "corner", "update", "ghosts", "reduce"]
+MLP_FEATURES = 100
+MLP_HIDDEN = 32
+MLP_PARAMS = 4321
+MLP_PACKED = 5632
+LABEL_MP_CLIP, LABEL_MUSCL, LABEL_CONSTANT, LABEL_HLL_GAP = 1, 2, 4, 8
+
+KERNEL_KINDS = ["prep", "reconstruct", "flux", "cell_emf", "corner", "update", "ghosts", "reduce", "indicator"]
 
 _lib = None
 
@@ -115,6 +126,22 @@ def load() -> C.CDLL:
     L.qmhd_solver_stream.argtypes = [VP]
     L.qmhd_solver_stream.restype = VP
     L.qmhd_solver_synchronize.argtypes = [VP]
+    I64 = C.c_int64
+    F = C.c_float
+    L.qmhd_mlp_pack.argtypes = [VP, VP, VP]
+    L.qmhd_mlp_forward.argtypes = [VP, VP, VP, I64, VP, VP]
+    L.qmhd_mlp_partial_floats.argtypes = [I64, C.c_int]
+    L.qmhd_mlp_partial_floats.restype = C.c_size_t
+    L.qmhd_mlp_gradient.argtypes = [VP, VP, VP, VP, VP, VP, I64, C.c_int, VP, VP, VP]
+    L.qmhd_mlp_adam.argtypes = [VP, VP, VP, VP, I64, F, F, F, F, VP]
+    L.qmhd_cell_features.argtypes = [C.POINTER(QGrid), D, VP, VP, VP, VP]
+    L.qmhd_cell_labels.argtypes = [C.POINTER(QGrid), C.c_int, I32P, D, VP, D, VP, VP, VP]
+    L.qmhd_cell_indicator.argtypes = [C.POINTER(QGrid), D, VP, VP, VP, F, VP, VP, VP]
+    L.qmhd_stream_synchronize.argtypes = [VP]
+    L.qmhd_solver_set_indicator.argtypes = [VP, VP, F]
+    L.qmhd_solver_plain_faces.argtypes = [VP, C.POINTER(I64), C.POINTER(I64)]
+    L.qmhd_solver_cell_features.argtypes = [VP, VP]
+    L.qmhd_solver_cell_labels.argtypes = [VP, D, VP]
     _lib = L
     return L
 

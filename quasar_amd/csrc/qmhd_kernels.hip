@@ -362,12 +362,81 @@ __global__ void counter_reset_kernel(unsigned long long* __restrict__ c) {
   if (threadIdx.x == 0) c[0] = 0ull;
 }
 
+// Learned form, one thread per face (the reconstruction launch domain): a
+// face whose two cells the indicator left unflagged takes the linear
+// fifth-order upwind-biased face values (the unlimited MP5 interpolant,
+// include/quasar/numerics/mp_limiter.hpp:182-191) of the conserved cell
+// averages, component by component, and the face normal field.  Flagged
+// faces, and cheap states that are not admissible, are listed for the plain
+// path (characteristic MP with its fallback chain).
+template <class T>
+__global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir, T gamma, StateView u,
+                                                                const double* __restrict__ bxc,
+                                                                const double* __restrict__ byc,
+                                                                const uint8_t* __restrict__ cell_flags,
+                                                                IfaceView<T> out, int* __restrict__ eq_flag,
+                                                                unsigned long long* __restrict__ fb_count,
+                                                                unsigned long long* __restrict__ fb_list) {
+  int i, j;
+  const bool valid = recon_face_of(g, dir, i, j);
+  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+  const std::size_t k = valid ? idx(g, i, j) : 0;
+  St<T> ul, ur;
+  bool need = false;
+  if (valid) {
+    need = (cell_flags[k] | cell_flags[idx(g, i - di, j - dj)]) != 0;
+    if (!need) {
+      St<T> c[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) c[q] = load_cell<T>(g, u, bxc, byc, i + (q - 3) * di, j + (q - 3) * dj);
+#pragma unroll
+      for (int cc = 0; cc < 8; ++cc) {
+        const T vl[5] = {comp(c[0], cc), comp(c[1], cc), comp(c[2], cc), comp(c[3], cc), comp(c[4], cc)};
+        const T vr[5] = {comp(c[5], cc), comp(c[4], cc), comp(c[3], cc), comp(c[2], cc), comp(c[1], cc)};
+        set_comp(ul, cc, mp5_face(vl));
+        set_comp(ur, cc, mp5_face(vr));
+      }
+      const T bn = T((dir == 0 ? u.c[5] : u.c[6])[k]);
+      set_normal_b(ul, dir, bn);
+      set_normal_b(ur, dir, bn);
+      need = !(admissible_state(ul, gamma) && admissible_state(ur, gamma));
+    }
+  }
+  const unsigned long long mask = __ballot(need);
+  if (mask != 0ull) {
+    const int lane = static_cast<int>(__lane_id());
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(fb_count, static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    if (need) fb_list[base + __popcll(mask & ((1ull << lane) - 1ull))] = k;
+  }
+  if (!valid || need) return;
+  write_faces(out, eq_flag, k, ul, ur);
+}
+
+// Running total of the faces sent to the plain path.
+__global__ void plain_faces_accumulate_kernel(const unsigned long long* __restrict__ fb_count,
+                                              unsigned long long* __restrict__ total) {
+  if (threadIdx.x == 0) total[0] += fb_count[0];
+}
+
 // ---------------------------------------------------------------------------
 // Face flux with transverse Gauss quadrature and the CFL face rate.
 
+// hll_only: the order-1 Lax-Friedrichs flux; cheap: the learned form's HLL
+// flux (Davis speeds); otherwise HLLD.
 template <class T>
-__device__ __forceinline__ St<T> riemann(const St<T>& L, const St<T>& R, int dir, int hll_only, T gamma) {
-  const St<T> f = hll_only ? lf_flux(rot(L, dir), rot(R, dir), gamma) : hlld_flux(rot(L, dir), rot(R, dir), gamma);
+__device__ __forceinline__ St<T> riemann(const St<T>& L, const St<T>& R, int dir, int hll_only, T gamma,
+                                         bool cheap = false) {
+  const St<T> lr = rot(L, dir), rr = rot(R, dir);
+  St<T> f;
+  if (hll_only)
+    f = lf_flux(lr, rr, gamma);
+  else if (cheap)
+    f = hll_davis_flux(lr, rr, gamma);
+  else
+    f = hlld_flux(lr, rr, gamma);
   return rot(f, dir);
 }
 
@@ -404,10 +473,15 @@ __device__ __forceinline__ T face_speed(const St<T>& L, const St<T>& R, int dir,
 
 // One thread per face in [0,nx] x [0,ny].  Writes the face flux (F_n of the
 // normal field is zero), and the face signal speed max(|v_n|)+max(c_f) over
-// the face average and every Gauss point, scaled by 0.5/spacing.
-template <class T, int ORDER>
+// the face average and every Gauss point, scaled by 0.5/spacing.  With
+// `cell_flags` (learned form) a face whose two cells are both unflagged takes
+// one HLL flux at its face-average states (second order across the face, as
+// accurate as needed where the indicator finds the flow smooth) and its rate
+// from those states.
+template <class T, int ORDER, bool LEARNED = false>
 __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_only, int periodic_dir, T gamma,
-                                                   IfaceView<T> in, FluxView<T> flux, double* __restrict__ rate) {
+                                                   IfaceView<T> in, FluxView<T> flux, double* __restrict__ rate,
+                                                   const uint8_t* __restrict__ cell_flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.nx || j > g.ny) return;
@@ -418,6 +492,8 @@ __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_onl
   }
   const std::size_t ko = idx(g, i, j);
   const std::size_t k = idx(g, si, sj);
+  bool cheap = false;
+  if (LEARNED) cheap = (cell_flags[k] | cell_flags[idx(g, si - (dir == 0), sj - (dir == 1))]) == 0;
   St<T> baseL, baseR;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
@@ -428,7 +504,10 @@ __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_onl
   St<T> f;
   T smax = face_speed(baseL, baseR, dir, ORDER, gamma);
   if (ORDER != 5 && ORDER != 7) {
-    f = riemann(baseL, baseR, dir, hll_only, gamma);
+    f = riemann(baseL, baseR, dir, hll_only, gamma, cheap);
+  } else if (LEARNED && cheap) {
+    // Learned form, smooth face: one HLL flux at the face-average states.
+    f = riemann(baseL, baseR, dir, hll_only, gamma, true);
   } else {
     constexpr int H = ORDER == 5 ? 2 : 3;
     constexpr int NQ = ORDER == 5 ? 3 : 4;
@@ -471,7 +550,7 @@ __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_onl
     for (int q = 0; q < NQ; ++q)
       if (q < first_bad) smax = fmax(smax, face_speed(pl[q], pr[q], dir, ORDER, gamma));
     if (!ok) {
-      f = riemann(baseL, baseR, dir, hll_only, gamma);
+      f = riemann(baseL, baseR, dir, hll_only, gamma, cheap);
     } else {
       // Gauss average of the point fluxes, accumulated point by point in the
       // same summation order, with the exact constant-flux shortcut (all
@@ -480,7 +559,7 @@ __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_onl
       bool same[8];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        const St<T> fq = riemann(pl[q], pr[q], dir, hll_only, gamma);
+        const St<T> fq = riemann(pl[q], pr[q], dir, hll_only, gamma, cheap);
         const T w = T(ORDER == 5 ? c_rows.mp5_gauss[q] : c_rows.mp7_gauss[q]);
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -533,10 +612,13 @@ __global__ __launch_bounds__(256) void cfl_reduce_kernel(GridD g, const double* 
 constexpr int kEmfBx = 16;
 constexpr int kEmfBy = 16;
 
-template <class T, int ORDER>
+// With LEARNED, a block none of whose cells the indicator flagged takes the
+// cell-average E_z (second order) for every cell.
+template <class T, int ORDER, bool LEARNED = false>
 __global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high, int phys_y_high, T gamma,
                                                        StateView u, const double* __restrict__ bxc,
-                                                       const double* __restrict__ byc, double* __restrict__ cell_ez) {
+                                                       const double* __restrict__ byc, double* __restrict__ cell_ez,
+                                                       const uint8_t* __restrict__ cell_flags) {
   constexpr int H = ORDER == 5 ? 2 : 3;
   constexpr int NQ = ORDER == 5 ? 3 : 4;
   constexpr int TW = 2 * H + 1;
@@ -565,6 +647,14 @@ __global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high,
   __syncthreads();
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
   const bool active = i <= g.nx && j <= g.ny && !(i == g.nx && !phys_x_high) && !(j == g.ny && !phys_y_high);
+  if (LEARNED && !__syncthreads_or(active && cell_flags[idx(g, i, j)] != 0)) {
+    if (active) {
+      const int a0 = threadIdx.x + H, b0 = threadIdx.y + H;
+      cell_ez[idx(g, i, j)] = static_cast<double>(
+          physical_ez(tile[0][b0][a0], tile[1][b0][a0], tile[2][b0][a0], tile[5][b0][a0], tile[6][b0][a0]));
+    }
+    return;
+  }
   // Stencil bounds inside the padded domain (always true for the computed
   // cells when ng >= H + 1).
   const bool in_range = i - H >= -g.ng && i + H <= g.nx + g.ng - 1 && j - H >= -g.ng && j + H <= g.ny + g.ng - 1;
@@ -1206,24 +1296,61 @@ void launch_reconstruct(const GridD& g, int dir, int order, double gamma, const 
 }
 
 template <class T>
+void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma, const Flags4& flags,
+                                const StateView& u, const double* bxc, const double* byc, const uint8_t* cell_flags,
+                                const IfaceView<T>& out, int* eq, unsigned long long* fb_count,
+                                unsigned long long* fb_list, unsigned long long* plain_total, hipStream_t stream) {
+  const dim3 b(64, 4);
+  const int lx = dir == 0 ? g.nx + 1 : g.nx + 2 * g.ng;
+  const int ly = dir == 0 ? g.ny + 2 * g.ng : g.ny + 1;
+  const int slo = dir == 0 ? flags.side[0] : flags.side[2];
+  const int shi = dir == 0 ? flags.side[1] : flags.side[3];
+  const T gamma_t = T(gamma);
+  // The plain list can hold a large share of the faces: enough blocks to fill
+  // the device at the MP kernel's occupancy.
+  const dim3 list_grid(1024), list_block(256);
+  hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb_count);
+  hipLaunchKernelGGL((cheap_reconstruct_kernel<T>), blocks2d(lx, ly, b), b, 0, stream, g, dir, gamma_t, u, bxc, byc,
+                     cell_flags, out, eq, fb_count, fb_list);
+  if (order == 5)
+    hipLaunchKernelGGL((reconstruct_fallback_kernel<T, 5>), list_grid, list_block, 0, stream, g, dir, gamma_t, slo, shi,
+                       u, bxc, byc, out, eq, fb_count, fb_list);
+  else
+    hipLaunchKernelGGL((reconstruct_fallback_kernel<T, 7>), list_grid, list_block, 0, stream, g, dir, gamma_t, slo, shi,
+                       u, bxc, byc, out, eq, fb_count, fb_list);
+  if (plain_total) hipLaunchKernelGGL(plain_faces_accumulate_kernel, dim3(1), dim3(64), 0, stream, fb_count, plain_total);
+}
+
+template <class T>
 void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4& flags, double gamma,
-                 const IfaceView<T>& in, const FluxView<T>& flux, double* rate, hipStream_t stream) {
+                 const IfaceView<T>& in, const FluxView<T>& flux, double* rate, hipStream_t stream,
+                 const uint8_t* cell_flags) {
   const T gm = T(gamma);
   const dim3 b(64, 4);
   const dim3 grid = blocks2d(g.nx + 1, g.ny + 1, b);
   const int periodic = dir == 0 ? (flags.side[0] == 0 && flags.side[1] == 0) : (flags.side[2] == 0 && flags.side[3] == 0);
   switch (order) {
     case 1:
-      hipLaunchKernelGGL((flux_kernel<T, 1>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate);
+      hipLaunchKernelGGL((flux_kernel<T, 1>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate, cell_flags);
       break;
     case 2:
-      hipLaunchKernelGGL((flux_kernel<T, 2>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate);
+      hipLaunchKernelGGL((flux_kernel<T, 2>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate, cell_flags);
       break;
     case 5:
-      hipLaunchKernelGGL((flux_kernel<T, 5>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate);
+      if (cell_flags)
+        hipLaunchKernelGGL((flux_kernel<T, 5, true>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate,
+                           cell_flags);
+      else
+        hipLaunchKernelGGL((flux_kernel<T, 5>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate,
+                           cell_flags);
       break;
     default:
-      hipLaunchKernelGGL((flux_kernel<T, 7>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate);
+      if (cell_flags)
+        hipLaunchKernelGGL((flux_kernel<T, 7, true>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate,
+                           cell_flags);
+      else
+        hipLaunchKernelGGL((flux_kernel<T, 7>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate,
+                           cell_flags);
       break;
   }
 }
@@ -1237,16 +1364,27 @@ void launch_cfl_reduce(const GridD& g, const double* rate_x, const double* rate_
 
 template <class T>
 void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamma, const StateView& u,
-                     const double* bxc, const double* byc, double* cell_ez, hipStream_t stream) {
+                     const double* bxc, const double* byc, double* cell_ez, hipStream_t stream,
+                     const uint8_t* cell_flags) {
   const int pxh = flags.side[1] != 0 && flags.side[1] != 4;
   const int pyh = flags.side[3] != 0 && flags.side[3] != 4;
   if (order == 5 || order == 7) {
     const dim3 b(kEmfBx, kEmfBy);
     const dim3 grid = blocks2d(g.nx + 1, g.ny + 1, b);
-    if (order == 5)
-      hipLaunchKernelGGL((cell_emf_kernel<T, 5>), grid, b, 0, stream, g, pxh, pyh, T(gamma), u, bxc, byc, cell_ez);
-    else
-      hipLaunchKernelGGL((cell_emf_kernel<T, 7>), grid, b, 0, stream, g, pxh, pyh, T(gamma), u, bxc, byc, cell_ez);
+    if (cell_flags) {
+      if (order == 5)
+        hipLaunchKernelGGL((cell_emf_kernel<T, 5, true>), grid, b, 0, stream, g, pxh, pyh, T(gamma), u, bxc, byc,
+                           cell_ez, cell_flags);
+      else
+        hipLaunchKernelGGL((cell_emf_kernel<T, 7, true>), grid, b, 0, stream, g, pxh, pyh, T(gamma), u, bxc, byc,
+                           cell_ez, cell_flags);
+    } else if (order == 5) {
+      hipLaunchKernelGGL((cell_emf_kernel<T, 5>), grid, b, 0, stream, g, pxh, pyh, T(gamma), u, bxc, byc, cell_ez,
+                         cell_flags);
+    } else {
+      hipLaunchKernelGGL((cell_emf_kernel<T, 7>), grid, b, 0, stream, g, pxh, pyh, T(gamma), u, bxc, byc, cell_ez,
+                         cell_flags);
+    }
   } else {
     const dim3 b(64, 4);
     hipLaunchKernelGGL(cell_emf_lo_kernel, blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, pxh, pyh,
@@ -1329,14 +1467,22 @@ hipError_t upload_quad_rows(const QuadRows& rows) {
                                       const double*, const double*, const IfaceView<T>&, int*, unsigned long long*, \
                                       unsigned long long*, hipStream_t);                                          \
   template void launch_flux<T>(const GridD&, int, int, int, const Flags4&, double, const IfaceView<T>&,            \
-                               const FluxView<T>&, double*, hipStream_t);                                         \
+                               const FluxView<T>&, double*, hipStream_t, const uint8_t*);                         \
   template void launch_cell_emf<T>(const GridD&, int, const Flags4&, double, const StateView&, const double*,     \
-                                   const double*, double*, hipStream_t);                                          \
+                                   const double*, double*, hipStream_t, const uint8_t*);                          \
   template void launch_corner<T>(const GridD&, int, const Flags4&, const FluxView<T>&, const FluxView<T>&,        \
                                  const double*, const int*, const int*, double*, hipStream_t);                    \
   template void launch_update<T>(const GridD&, double, double, double, const double*, const StateView&,           \
                                  const StateView&, const FluxView<T>&, const FluxView<T>&, const double*,         \
                                  const StateOut&, hipStream_t);
+#define QMHD_INSTANTIATE_LEARNED(T)                                                                                \
+  template void launch_reconstruct_learned<T>(const GridD&, int, int, double, const Flags4&, const StateView&,     \
+                                              const double*, const double*, const uint8_t*, const IfaceView<T>&,  \
+                                              int*, unsigned long long*, unsigned long long*, unsigned long long*, \
+                                              hipStream_t);
+QMHD_INSTANTIATE_LEARNED(double)
+QMHD_INSTANTIATE_LEARNED(float)
+#undef QMHD_INSTANTIATE_LEARNED
 QMHD_INSTANTIATE(double)
 QMHD_INSTANTIATE(float)
 #undef QMHD_INSTANTIATE

@@ -77,14 +77,24 @@ template <class T>
 void launch_reconstruct(const GridD& g, int dir, int order, double gamma, const Flags4& flags, const StateView& u,
                         const double* bxc, const double* byc, const IfaceView<T>& out, int* eq,
                         unsigned long long* fb_count, unsigned long long* fb_list, hipStream_t stream);
+// Learned form (orders 5 and 7): cheap faces where the indicator left both
+// cells unflagged, the plain path over the list of the others; adds the
+// listed count to *plain_total when given.
+template <class T>
+void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma, const Flags4& flags,
+                                const StateView& u, const double* bxc, const double* byc, const uint8_t* cell_flags,
+                                const IfaceView<T>& out, int* eq, unsigned long long* fb_count,
+                                unsigned long long* fb_list, unsigned long long* plain_total, hipStream_t stream);
 template <class T>
 void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4& flags, double gamma,
-                 const IfaceView<T>& in, const FluxView<T>& flux, double* rate, hipStream_t stream);
+                 const IfaceView<T>& in, const FluxView<T>& flux, double* rate, hipStream_t stream,
+                 const uint8_t* cell_flags = nullptr);
 void launch_cfl_reduce(const GridD& g, const double* rate_x, const double* rate_y, unsigned long long* max_rate,
                        hipStream_t stream);
 template <class T>
 void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamma, const StateView& u,
-                     const double* bxc, const double* byc, double* cell_ez, hipStream_t stream);
+                     const double* bxc, const double* byc, double* cell_ez, hipStream_t stream,
+                     const uint8_t* cell_flags = nullptr);
 template <class T>
 void launch_corner(const GridD& g, int order, const Flags4& flags, const FluxView<T>& fx, const FluxView<T>& fy,
                    const double* cell_ez, const int* eqx, const int* eqy, double* ez, hipStream_t stream);
@@ -108,5 +118,28 @@ void launch_flux_difference(const GridD& g, int dir, const FluxView<double>& f, 
                             hipStream_t stream);
 void launch_curl(const GridD& g, const double* ez, double* dbx, double* dby, hipStream_t stream);
 void launch_status_reset(StepStatus* s, hipStream_t stream);
+
+// Learned form: troubled-cell indicator network (parameters: FP32 vector of
+// mlp::kParams; packed: BF16 MFMA operands from launch_mlp_pack), training
+// kernels, features and labels.
+void launch_mlp_pack(const float* params, uint16_t* packed, hipStream_t stream);
+void launch_mlp_forward(const uint16_t* packed, const float* params, const float* x, long long n, float* logits,
+                        hipStream_t stream);
+long long mlp_grad_waves(long long n, int chunks);
+// grad receives mlp::kParams gradient entries and the loss; partial holds
+// mlp_grad_waves(n, chunks) rounded up to 4 rows of mlp::kParams + 1 floats.
+void launch_mlp_grad(const uint16_t* packed, const float* params, const float* x, const float* target,
+                     const float* weight, const long long* order, long long n, int chunks, float* partial,
+                     float* grad, hipStream_t stream);
+void launch_mlp_adam(float* params, const float* grad, float* m, float* v, long long step, float lr, float beta1,
+                     float beta2, float eps, hipStream_t stream);
+void launch_features(const GridD& g, double gamma, const StateView& u, const double* bxc, const double* byc,
+                     float* out, hipStream_t stream);
+void launch_indicator(const GridD& g, double gamma, const StateView& u, const double* bxc, const double* byc,
+                      const uint16_t* packed, const float* params, float threshold, uint8_t* flags,
+                      hipStream_t stream);
+void launch_labels(const GridD& g, int order, const Flags4& flags, double gamma, const StateView& u,
+                   const double* bxc, const double* byc, double label_tol, uint8_t* fx, uint8_t* fy, uint8_t* cell,
+                   hipStream_t stream);
 
 }  // namespace qmhd

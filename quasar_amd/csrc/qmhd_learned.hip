@@ -1,0 +1,448 @@
+// Learned form: troubled-cell indicator (features + BF16 MFMA network), its
+// training kernels (forward, backward, deterministic gradient reduction,
+// Adam) and the label kernels that mark where the plain method's MP limiter,
+// fallback chain or HLLD-vs-HLL flux gap is active.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "qmhd_kernels.hpp"
+#include "qmhd_mlp.hpp"
+#include "qmhd_stencil.hpp"
+
+namespace qmhd {
+
+using mlp::bf16x4;
+using mlp::f32x4;
+
+namespace {
+
+inline dim3 grid2d(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+// ---------------------------------------------------------------------------
+// Network kernels.
+
+__global__ __launch_bounds__(256) void mlp_pack_kernel(const float* __restrict__ p, uint16_t* __restrict__ packed) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < mlp::kPackTotal) packed[t] = mlp::packed_value(p, t);
+}
+
+// B operands of layer 1 for the 16 samples rows[0..15] of a row-major
+// (n x 100) FP32 feature matrix; a negative row is an empty slot.
+__device__ __forceinline__ void load_features16(const float* __restrict__ x, long long row, bf16x4 xb[mlp::kKs1]) {
+  const int grp = static_cast<int>(__lane_id()) >> 4;
+#pragma unroll
+  for (int ks = 0; ks < mlp::kKs1; ++ks) {
+    const int f0 = 16 * ks + 4 * grp;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (row >= 0 && f0 < mlp::kIn) v = *reinterpret_cast<const float4*>(x + row * mlp::kIn + f0);
+    xb[ks] = mlp::pack4(v.x, v.y, v.z, v.w);
+  }
+}
+
+// Logits of n samples, 16 per wave-pass.
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const uint16_t* __restrict__ packed,
+                                                          const float* __restrict__ p, const float* __restrict__ x,
+                                                          long long n, float* __restrict__ logits) {
+  const int lane = static_cast<int>(__lane_id());
+  const long long waves = static_cast<long long>(gridDim.x) * (blockDim.x / 64);
+  const long long wave = static_cast<long long>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
+  const bf16x4* pk = reinterpret_cast<const bf16x4*>(packed);
+  for (long long base = wave * 16; base < n; base += waves * 16) {
+    const long long s = base + (lane & 15);
+    bf16x4 xb[mlp::kKs1];
+    load_features16(x, s < n ? s : -1, xb);
+    f32x4 z1[2], z2[2];
+    const float z = mlp::forward16(pk, p, xb, z1, z2);
+    if (lane < 16 && s < n) logits[s] = z;
+  }
+}
+
+constexpr int kGradWaves = 4;
+constexpr int kPartialStride = mlp::kParams + 1;  // parameters, then the loss
+
+__device__ __forceinline__ float group_sum16(float v) {
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  v += __shfl_xor(v, 8, 64);
+  return v;
+}
+
+__device__ __forceinline__ bf16x4 lds4(const uint16_t* a) {
+  const uint2 v = *reinterpret_cast<const uint2*>(a);
+  bf16x4 r;
+  r[0] = static_cast<short>(v.x & 0xffffu);
+  r[1] = static_cast<short>(v.x >> 16);
+  r[2] = static_cast<short>(v.y & 0xffffu);
+  r[3] = static_cast<short>(v.y >> 16);
+  return r;
+}
+
+// Weighted binary cross-entropy on the logits and its gradient for the
+// samples order[0..n) (identity when order is null), `chunks` passes of 16
+// samples per wave.  Each wave writes its partial gradient (and loss) to
+// partial[wave]; the products of the backward pass run on MFMA with BF16
+// operands: dW2 = H1 d2^T, d1 = W2 d2 and dW1 = X^T d1 with H1, d2, d1 and X
+// rounded to BF16 as the forward pass rounds them.
+__global__ __launch_bounds__(256) void mlp_grad_kernel(const uint16_t* __restrict__ packed,
+                                                       const float* __restrict__ p, const float* __restrict__ x,
+                                                       const float* __restrict__ target,
+                                                       const float* __restrict__ weight,
+                                                       const long long* __restrict__ order, long long n, int chunks,
+                                                       float* __restrict__ partial) {
+  using namespace mlp;
+  __shared__ __align__(16) uint16_t xs_t[kGradWaves][kInPad][16];
+  __shared__ __align__(16) uint16_t h1s[kGradWaves][kH][16];
+  __shared__ __align__(16) uint16_t d2s[kGradWaves][kH][16];
+  __shared__ __align__(16) uint16_t d1s[kGradWaves][kH][16];
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, grp = lane >> 4, col = lane & 15;
+  const long long wave = static_cast<long long>(blockIdx.x) * kGradWaves + wv;
+  const bf16x4* pk = reinterpret_cast<const bf16x4*>(packed);
+  const bf16x4* l2t = pk + (kPackL1 + kPackL2) / 4;
+  f32x4 gw1[kKs1][2], gw2[2][2];
+#pragma unroll
+  for (int a = 0; a < kKs1; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) gw1[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) gw2[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float gb1[2][4] = {}, gb2[2][4] = {}, gw3[2][4] = {};
+  float gb3 = 0.f, loss = 0.f;
+  for (int c = 0; c < chunks; ++c) {
+    const long long s = (wave * chunks + c) * 16 + col;
+    const bool live = s < n;
+    const long long row = live ? (order ? order[s] : s) : -1;
+    bf16x4 xb[kKs1];
+    load_features16(x, row, xb);
+#pragma unroll
+    for (int ks = 0; ks < kKs1; ++ks)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) xs_t[wv][16 * ks + 4 * grp + e][col] = static_cast<uint16_t>(xb[ks][e]);
+    f32x4 z1[2], z2[2];
+    const float z = forward16(pk, p, xb, z1, z2);
+    const float yt = live ? target[row] : 0.f;
+    const float wt = live ? weight[row] : 0.f;
+    const float sig = 1.f / (1.f + __expf(-z));
+    const float dl = wt * (sig - yt);
+    if (grp == 0) {
+      gb3 += dl;
+      if (live) loss += wt * (fmaxf(z, 0.f) - z * yt + log1pf(__expf(-fabsf(z))));
+    }
+    float d2[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int h = 16 * t + 4 * grp + e;
+        const float a2 = fmaxf(z2[t][e], 0.f);
+        gw3[t][e] += a2 * dl;
+        d2[t][e] = z2[t][e] > 0.f ? p[kOffW3 + h] * dl : 0.f;
+        gb2[t][e] += d2[t][e];
+        h1s[wv][h][col] = f2bf(fmaxf(z1[t][e], 0.f));
+        d2s[wv][h][col] = f2bf(d2[t][e]);
+      }
+    bf16x4 d2b[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) d2b[t] = pack4(d2[t][0], d2[t][1], d2[t][2], d2[t][3]);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) acc = mfma16(l2t[(t * 2 + ks) * 64 + lane], d2b[ks], acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float d1 = z1[t][e] > 0.f ? acc[e] : 0.f;
+        gb1[t][e] += d1;
+        d1s[wv][16 * t + 4 * grp + e][col] = f2bf(d1);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t1 = 0; t1 < 2; ++t1) {
+      const bf16x4 a = lds4(&h1s[wv][16 * t1 + col][4 * grp]);
+#pragma unroll
+      for (int t2 = 0; t2 < 2; ++t2) gw2[t1][t2] = mfma16(a, lds4(&d2s[wv][16 * t2 + col][4 * grp]), gw2[t1][t2]);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const bf16x4 bd = lds4(&d1s[wv][16 * t + col][4 * grp]);
+#pragma unroll
+      for (int ft = 0; ft < kKs1; ++ft) gw1[ft][t] = mfma16(lds4(&xs_t[wv][16 * ft + col][4 * grp]), bd, gw1[ft][t]);
+    }
+    __syncthreads();
+  }
+  float* out = partial + wave * kPartialStride;
+#pragma unroll
+  for (int ft = 0; ft < kKs1; ++ft)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int f = 16 * ft + 4 * grp + e;
+        if (f < kIn) out[kOffW1 + f * kH + 16 * t + col] = gw1[ft][t][e];
+      }
+#pragma unroll
+  for (int t1 = 0; t1 < 2; ++t1)
+#pragma unroll
+    for (int t2 = 0; t2 < 2; ++t2)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) out[kOffW2 + (16 * t1 + 4 * grp + e) * kH + 16 * t2 + col] = gw2[t1][t2][e];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float s1 = group_sum16(gb1[t][e]), s2 = group_sum16(gb2[t][e]), s3 = group_sum16(gw3[t][e]);
+      if (col == 0) {
+        const int h = 16 * t + 4 * grp + e;
+        out[kOffB1 + h] = s1;
+        out[kOffB2 + h] = s2;
+        out[kOffW3 + h] = s3;
+      }
+    }
+  const float sb3 = group_sum16(gb3), sl = group_sum16(loss);
+  if (lane == 0) {
+    out[kOffB3] = sb3;
+    out[kParams] = sl;
+  }
+}
+
+// grad[k] = sum over waves of partial[wave][k] in wave order (deterministic);
+// grad[kParams] is the loss.
+__global__ __launch_bounds__(256) void mlp_reduce_kernel(const float* __restrict__ partial, long long waves,
+                                                         float* __restrict__ grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= kPartialStride) return;
+  float s = 0.f;
+  for (long long w = 0; w < waves; ++w) s += partial[w * kPartialStride + k];
+  grad[k] = s;
+}
+
+__global__ __launch_bounds__(256) void mlp_adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                                                       float* __restrict__ m, float* __restrict__ v, float lr,
+                                                       float beta1, float beta2, float eps, float c1, float c2) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= mlp::kParams) return;
+  const float gk = g[k];
+  const float mk = beta1 * m[k] + (1.f - beta1) * gk;
+  const float vk = beta2 * v[k] + (1.f - beta2) * gk * gk;
+  m[k] = mk;
+  v[k] = vk;
+  p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Features and the indicator.
+
+__device__ __forceinline__ mlp::CellQ quantities_at(const GridD& g, const StateView& u, const double* bxc,
+                                                    const double* byc, double gamma, int i, int j) {
+  const std::size_t k = idx(g, i, j);
+  return mlp::cell_quantities(u.c[0][k], u.c[1][k], u.c[2][k], u.c[3][k], u.c[4][k], bxc[k], byc[k], u.c[7][k], gamma);
+}
+
+// Features of every interior cell, row-major (nx*ny x 100), cell (i,j) at
+// row j*nx + i.
+__global__ __launch_bounds__(256) void features_kernel(GridD g, double gamma, StateView u,
+                                                       const double* __restrict__ bxc, const double* __restrict__ byc,
+                                                       float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  mlp::CellQ s[5][5];
+  for (int dy = 0; dy < 5; ++dy)
+    for (int dx = 0; dx < 5; ++dx) s[dy][dx] = quantities_at(g, u, bxc, byc, gamma, i + dx - 2, j + dy - 2);
+  float* o = out + (static_cast<std::size_t>(j) * g.nx + i) * mlp::kIn;
+  for (int q = 0; q < 4; ++q) {
+    const float sc = mlp::feature_scale([&](int dy, int dx) { return s[dy + 2][dx + 2].q[q]; });
+    for (int dy = 0; dy < 5; ++dy)
+      for (int dx = 0; dx < 5; ++dx) o[25 * q + 5 * dy + dx] = mlp::feature_value(s[dy][dx].q[q], s[2][2].q[q], sc);
+  }
+}
+
+constexpr int kIndTile = 16;
+constexpr int kIndSpan = kIndTile + 2 * mlp::kHalo;
+
+// One 16x16 tile of padded cells per block: quantities with a 2-cell halo in
+// LDS, per-cell feature scales, then 4 passes of 16 cells per wave through the
+// network.  flags = 1 (plain path) when the logit reaches `threshold`, or the
+// cell's stencil leaves the padded domain or is not finite.
+__global__ __launch_bounds__(256) void indicator_kernel(GridD g, double gamma, StateView u,
+                                                        const double* __restrict__ bxc,
+                                                        const double* __restrict__ byc,
+                                                        const uint16_t* __restrict__ packed,
+                                                        const float* __restrict__ p, float threshold,
+                                                        uint8_t* __restrict__ flags) {
+  __shared__ float q[4][kIndSpan][kIndSpan];
+  __shared__ float sc[4][kIndTile][kIndTile];
+  __shared__ int ok[kIndTile][kIndTile];
+  const int i0 = static_cast<int>(blockIdx.x) * kIndTile - g.ng;
+  const int j0 = static_cast<int>(blockIdx.y) * kIndTile - g.ng;
+  const int tid = threadIdx.x;
+  const float nan = __builtin_nanf("");
+  for (int t = tid; t < kIndSpan * kIndSpan; t += 256) {
+    const int a = t % kIndSpan, b = t / kIndSpan;
+    const int ii = i0 + a - mlp::kHalo, jj = j0 + b - mlp::kHalo;
+    const bool inside = ii >= -g.ng && ii < g.nx + g.ng && jj >= -g.ng && jj < g.ny + g.ng;
+    mlp::CellQ c;
+    if (inside) c = quantities_at(g, u, bxc, byc, gamma, ii, jj);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) q[m][b][a] = inside ? c.q[m] : nan;
+  }
+  __syncthreads();
+  {
+    const int tx = tid % kIndTile, ty = tid / kIndTile;
+    bool good = true;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const float s = mlp::feature_scale([&](int dy, int dx) { return q[m][ty + 2 + dy][tx + 2 + dx]; });
+      sc[m][ty][tx] = s;
+      good = good && __builtin_isfinite(s);
+      for (int dy = -2; dy <= 2; ++dy)
+        for (int dx = -2; dx <= 2; ++dx) good = good && __builtin_isfinite(q[m][ty + 2 + dy][tx + 2 + dx]);
+    }
+    ok[ty][tx] = good ? 1 : 0;
+  }
+  __syncthreads();
+  const int wv = tid >> 6, lane = tid & 63, grp = lane >> 4, col = lane & 15;
+  const bf16x4* pk = reinterpret_cast<const bf16x4*>(packed);
+  for (int r = 0; r < 4; ++r) {
+    const int row = 4 * wv + r;
+    bf16x4 xb[mlp::kKs1];
+#pragma unroll
+    for (int ks = 0; ks < mlp::kKs1; ++ks) {
+      float v[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int f = 16 * ks + 4 * grp + e;
+        v[e] = 0.f;
+        if (f < mlp::kIn) {
+          int m, dy, dx;
+          mlp::feature_index(f, m, dy, dx);
+          v[e] = mlp::feature_value(q[m][row + 2 + dy][col + 2 + dx], q[m][row + 2][col + 2], sc[m][row][col]);
+        }
+      }
+      xb[ks] = mlp::pack4(v[0], v[1], v[2], v[3]);
+    }
+    f32x4 z1[2], z2[2];
+    const float z = mlp::forward16(pk, p, xb, z1, z2);
+    const int i = i0 + col, j = j0 + row;
+    if (grp == 0 && i < g.nx + g.ng && j < g.ny + g.ng)
+      flags[idx(g, i, j)] = (ok[row][col] && z < threshold) ? 0 : 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Labels: path bits of every face adjacent to an interior cell, from the
+// plain FP64 method (MP limiter activity, fallback chain), plus the HLLD-vs-HLL
+// gap at the face-average states; cell bits are the OR of the four faces.
+
+template <int ORDER>
+__global__ __launch_bounds__(256) void face_label_kernel(GridD g, int dir, double gamma, int side_lo, int side_hi,
+                                                         StateView u, const double* __restrict__ bxc,
+                                                         const double* __restrict__ byc, double label_tol,
+                                                         uint8_t* __restrict__ bits_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (dir == 0 ? (i > g.nx || j >= g.ny) : (i >= g.nx || j > g.ny)) return;
+  LabelProbe probe{0u, label_tol};
+  St<double> ul, ur;
+  reconstruct_face<double, ORDER>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur, &probe);
+  unsigned bits = probe.bits;
+  if (!(bits & kPathConstant) || (admissible_state(ul, gamma) && admissible_state(ur, gamma))) {
+    const St<double> lr = rot(ul, dir), rr = rot(ur, dir);
+    const St<double> fd = hlld_flux(lr, rr, gamma);
+    const St<double> fh = hll_davis_flux(lr, rr, gamma);
+    double gap = 0.0, mag = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      gap = fmax(gap, __builtin_fabs(comp(fd, c) - comp(fh, c)));
+      mag = fmax(mag, __builtin_fabs(comp(fd, c)));
+    }
+    if (!(gap <= label_tol * (mag + 1e-300))) bits |= kPathHllGap;
+  }
+  bits_out[idx(g, i, j)] = static_cast<uint8_t>(bits);
+}
+
+__global__ __launch_bounds__(256) void cell_label_kernel(GridD g, const uint8_t* __restrict__ fx,
+                                                         const uint8_t* __restrict__ fy, uint8_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  out[static_cast<std::size_t>(j) * g.nx + i] =
+      fx[idx(g, i, j)] | fx[idx(g, i + 1, j)] | fy[idx(g, i, j)] | fy[idx(g, i, j + 1)];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers.
+
+void launch_mlp_pack(const float* params, uint16_t* packed, hipStream_t stream) {
+  hipLaunchKernelGGL(mlp_pack_kernel, dim3((mlp::kPackTotal + 255) / 256), dim3(256), 0, stream, params, packed);
+}
+
+void launch_mlp_forward(const uint16_t* packed, const float* params, const float* x, long long n, float* logits,
+                        hipStream_t stream) {
+  const long long waves = (n + 15) / 16;
+  const long long blocks = std::min<long long>((waves + 3) / 4, 4096);
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3(static_cast<unsigned>(std::max<long long>(blocks, 1))), dim3(256), 0,
+                     stream, packed, params, x, n, logits);
+}
+
+long long mlp_grad_waves(long long n, int chunks) { return ((n + 15) / 16 + chunks - 1) / chunks; }
+
+void launch_mlp_grad(const uint16_t* packed, const float* params, const float* x, const float* target,
+                     const float* weight, const long long* order, long long n, int chunks, float* partial,
+                     float* grad, hipStream_t stream) {
+  const long long waves = mlp_grad_waves(n, chunks);
+  const long long blocks = (waves + kGradWaves - 1) / kGradWaves;
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(static_cast<unsigned>(blocks)), dim3(64 * kGradWaves), 0, stream, packed,
+                     params, x, target, weight, order, n, chunks, partial);
+  hipLaunchKernelGGL(mlp_reduce_kernel, dim3((kPartialStride + 255) / 256), dim3(256), 0, stream, partial,
+                     blocks * kGradWaves, grad);
+}
+
+void launch_mlp_adam(float* params, const float* grad, float* m, float* v, long long step, float lr, float beta1,
+                     float beta2, float eps, hipStream_t stream) {
+  const float c1 = 1.f - std::pow(beta1, static_cast<float>(step));
+  const float c2 = 1.f - std::pow(beta2, static_cast<float>(step));
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((mlp::kParams + 255) / 256), dim3(256), 0, stream, params, grad, m, v, lr,
+                     beta1, beta2, eps, c1, c2);
+}
+
+void launch_features(const GridD& g, double gamma, const StateView& u, const double* bxc, const double* byc,
+                     float* out, hipStream_t stream) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(features_kernel, grid2d(g.nx, g.ny, b), b, 0, stream, g, gamma, u, bxc, byc, out);
+}
+
+void launch_indicator(const GridD& g, double gamma, const StateView& u, const double* bxc, const double* byc,
+                      const uint16_t* packed, const float* params, float threshold, uint8_t* flags,
+                      hipStream_t stream) {
+  const dim3 grid((g.nx + 2 * g.ng + kIndTile - 1) / kIndTile, (g.ny + 2 * g.ng + kIndTile - 1) / kIndTile);
+  hipLaunchKernelGGL(indicator_kernel, grid, dim3(256), 0, stream, g, gamma, u, bxc, byc, packed, params, threshold,
+                     flags);
+}
+
+void launch_labels(const GridD& g, int order, const Flags4& flags, double gamma, const StateView& u,
+                   const double* bxc, const double* byc, double label_tol, uint8_t* fx, uint8_t* fy, uint8_t* cell,
+                   hipStream_t stream) {
+  const dim3 b(64, 4);
+  const dim3 grid = grid2d(g.nx + 1, g.ny + 1, b);
+  for (int dir = 0; dir < 2; ++dir) {
+    const int slo = dir == 0 ? flags.side[0] : flags.side[2];
+    const int shi = dir == 0 ? flags.side[1] : flags.side[3];
+    uint8_t* out = dir == 0 ? fx : fy;
+    if (order == 5)
+      hipLaunchKernelGGL(face_label_kernel<5>, grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, label_tol, out);
+    else
+      hipLaunchKernelGGL(face_label_kernel<7>, grid, b, 0, stream, g, dir, gamma, slo, shi, u, bxc, byc, label_tol, out);
+  }
+  hipLaunchKernelGGL(cell_label_kernel, grid2d(g.nx, g.ny, b), b, 0, stream, g, fx, fy, cell);
+}
+
+}  // namespace qmhd

@@ -1,0 +1,185 @@
+// Per-cell troubled-cell indicator of the learned form: the cell features and
+// a 100-32-32-1 ReLU network evaluated with BF16 MFMA (v_mfma_f32_16x16x16_bf16,
+// FP32 accumulate).  Shared by the solver's indicator kernel, the training
+// kernels and the host build used by the CPU tests.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "qmhd_kernels.hpp"
+
+namespace qmhd {
+namespace mlp {
+
+constexpr int kIn = 100;     // 5x5 stencil x (rho, p, |B|, |v|)
+constexpr int kInPad = 112;  // padded to 7 k-steps of 16
+constexpr int kH = 32;
+constexpr int kKs1 = kInPad / 16;
+// FP32 parameter vector: W1[f][h], b1[h], W2[h1][h2], b2[h2], w3[h], b3.
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kIn * kH;
+constexpr int kOffW2 = kOffB1 + kH;
+constexpr int kOffB2 = kOffW2 + kH * kH;
+constexpr int kOffW3 = kOffB2 + kH;
+constexpr int kOffB3 = kOffW3 + kH;
+constexpr int kParams = kOffB3 + 1;
+// Packed BF16 A operands in lane order (4 values per lane per MFMA):
+// layer 1 W1^T tiles [2][7][64][4], layer 2 W2^T tiles [2][2][64][4] and the
+// backward W2 tiles [2][2][64][4].
+constexpr int kPackL1 = 2 * kKs1 * 64 * 4;
+constexpr int kPackL2 = 2 * 2 * 64 * 4;
+constexpr int kPackL2T = kPackL2;
+constexpr int kPackTotal = kPackL1 + kPackL2 + kPackL2T;
+// Features need the 5x5 stencil.
+constexpr int kHalo = 2;
+
+// Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).
+__host__ __device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u;
+  __builtin_memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf2f(uint16_t h) {
+  const uint32_t u = static_cast<uint32_t>(h) << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+// Value of packed operand element `e` of `lane` (FP32 parameters in, the BF16
+// bit pattern out).  Layer 1: A[i=h][k=f] = W1[f][h] with h = 16t + lane%16,
+// f = 16ks + 4(lane/16) + e.  Layer 2: A[i=h2][k=h1] = W2[h1][h2].  Backward:
+// A[i=h1][k=h2] = W2[h1][h2].
+__host__ __device__ __forceinline__ uint16_t packed_value(const float* p, int index) {
+  int r = index;
+  if (r < kPackL1) {
+    const int e = r & 3, lane = (r >> 2) & 63, tk = r >> 8;
+    const int t = tk / kKs1, ks = tk % kKs1;
+    const int h = 16 * t + (lane & 15), f = 16 * ks + 4 * (lane >> 4) + e;
+    return f < kIn ? f2bf(p[kOffW1 + f * kH + h]) : uint16_t(0);
+  }
+  r -= kPackL1;
+  const bool transposed = r >= kPackL2;
+  if (transposed) r -= kPackL2;
+  const int e = r & 3, lane = (r >> 2) & 63, tk = r >> 8;
+  const int t = tk / 2, ks = tk % 2;
+  const int row = 16 * t + (lane & 15), col = 16 * ks + 4 * (lane >> 4) + e;
+  // Layer 2: row = h2, col = h1 -> W2[h1][h2].  Backward: row = h1, col = h2.
+  return transposed ? f2bf(p[kOffW2 + row * kH + col]) : f2bf(p[kOffW2 + col * kH + row]);
+}
+
+// ---------------------------------------------------------------------------
+// Cell quantities and features.  The quantities are evaluated in FP64 and
+// rounded to FP32; the normalisation runs in FP32.  Contraction is off and
+// FP32 division is correctly rounded, so the oracle's evaluation of the same
+// expressions agrees bit for bit.
+
+struct CellQ {
+  float q[4];  // rho, p, |B|, |v|
+};
+
+__host__ __device__ __forceinline__ CellQ cell_quantities(double rho, double mx, double my, double mz, double en,
+                                                          double bx, double by, double bz, double gamma) {
+#pragma clang fp contract(off)
+  CellQ c;
+  const double m2 = (mx * mx + my * my) + mz * mz;
+  const double b2 = (bx * bx + by * by) + bz * bz;
+  c.q[0] = static_cast<float>(rho);
+  c.q[1] = static_cast<float>((gamma - 1.0) * ((en - 0.5 * m2 / rho) - 0.5 * b2));
+  c.q[2] = static_cast<float>(__builtin_sqrt(b2));
+  c.q[3] = static_cast<float>(__builtin_sqrt(m2) / rho);
+  return c;
+}
+
+// 1 / (|Q_c| + max_k |Q_k - Q_c| + tiny) over the 5x5 stencil of one quantity,
+// `at(dy, dx)` returning the stencil values.
+template <class F>
+__host__ __device__ __forceinline__ float feature_scale(F at) {
+#pragma clang fp contract(off)
+  const float c = at(0, 0);
+  float dev = 0.f;
+  for (int dy = -kHalo; dy <= kHalo; ++dy)
+    for (int dx = -kHalo; dx <= kHalo; ++dx) dev = fmaxf(dev, __builtin_fabsf(at(dy, dx) - c));
+  return 1.f / ((__builtin_fabsf(c) + dev) + 1e-30f);
+}
+
+// Feature f = 25 q + 5 (dy + 2) + (dx + 2): (Q_q(dy,dx) - Q_q(0,0)) * scale_q.
+__host__ __device__ __forceinline__ void feature_index(int f, int& q, int& dy, int& dx) {
+  q = f / 25;
+  const int r = f - 25 * q;
+  dy = r / 5 - kHalo;
+  dx = r - 5 * (r / 5) - kHalo;
+}
+
+__host__ __device__ __forceinline__ float feature_value(float qk, float qc, float scale) {
+#pragma clang fp contract(off)
+  return (qk - qc) * scale;
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level network (device code).
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+  bf16x4 v;
+  v[0] = static_cast<short>(f2bf(a));
+  v[1] = static_cast<short>(f2bf(b));
+  v[2] = static_cast<short>(f2bf(c));
+  v[3] = static_cast<short>(f2bf(d));
+  return v;
+}
+
+// Forward pass of the 16 samples one wave holds.  xb[ks] is the B operand of
+// layer 1 (features f = 16ks + 4(lane/16) + e of sample lane%16).  Returns
+// the logit of sample lane%16 in every lane; z1/z2 receive the pre-activation
+// tiles of both hidden layers in the MFMA output layout (hidden unit
+// 16t + 4(lane/16) + e of sample lane%16).
+__device__ __forceinline__ float forward16(const bf16x4* __restrict__ pk, const float* __restrict__ p,
+                                           const bf16x4 xb[kKs1], f32x4 z1[2], f32x4 z2[2]) {
+  const int lane = static_cast<int>(__lane_id());
+  const int grp = lane >> 4;
+  const bf16x4* l1 = pk;
+  const bf16x4* l2 = pk + kPackL1 / 4;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kKs1; ++ks) acc = mfma16(l1[(t * kKs1 + ks) * 64 + lane], xb[ks], acc);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] += p[kOffB1 + 16 * t + 4 * grp + e];
+    z1[t] = acc;
+  }
+  bf16x4 hb[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+    hb[t] = pack4(fmaxf(z1[t][0], 0.f), fmaxf(z1[t][1], 0.f), fmaxf(z1[t][2], 0.f), fmaxf(z1[t][3], 0.f));
+  float part = 0.f;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) acc = mfma16(l2[(t * 2 + ks) * 64 + lane], hb[ks], acc);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      acc[e] += p[kOffB2 + 16 * t + 4 * grp + e];
+      part += p[kOffW3 + 16 * t + 4 * grp + e] * fmaxf(acc[e], 0.f);
+    }
+    z2[t] = acc;
+  }
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  return part + p[kOffB3];
+}
+
+}  // namespace mlp
+}  // namespace qmhd

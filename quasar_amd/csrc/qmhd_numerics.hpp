@@ -137,8 +137,10 @@ __host__ __device__ __forceinline__ T minmod4(T a, T b, T c, T d) {
                    fmin(qabs(c), qabs(d)));
 }
 
+// `shift_max` (optional) receives the largest |limited - unlimited| seen
+// (the label of the troubled-cell indicator).
 template <class T>
-__host__ __device__ __forceinline__ T mp_limit(T vl, T vm2, T vm1, T v0, T vp1, T vp2) {
+__host__ __device__ __forceinline__ T mp_limit(T vl, T vm2, T vm1, T v0, T vp1, T vp2, T* shift_max = nullptr) {
   const T d_minus = v0 - vm1;
   const T d_plus = vp1 - v0;
   const T vmp = v0 + minmod2(d_plus, T(4) * d_minus);
@@ -161,7 +163,9 @@ __host__ __device__ __forceinline__ T mp_limit(T vl, T vm2, T vm1, T v0, T vp1, 
   const T v_lc = v0 + T(0.5) * d_minus + (T(4) / T(3)) * djmh;
   const T vmin = fmax(fmin(fmin(v0, vp1), v_md), fmin(fmin(v0, v_ul), v_lc));
   const T vmax = fmin(fmax(fmax(v0, vp1), v_md), fmax(fmax(v0, v_ul), v_lc));
-  return vl + minmod2(vmin - vl, vmax - vl);
+  const T shift = minmod2(vmin - vl, vmax - vl);
+  if (shift_max) *shift_max = fmax(*shift_max, qabs(shift));
+  return vl + shift;
 }
 
 // Finite-volume left-biased face values: MP5 (2,-13,47,27,-3)/60 and
@@ -652,6 +656,29 @@ __host__ __device__ __forceinline__ St<T> lf_flux(const St<T>& L, const St<T>& R
   const T aR = qabs(R.mx / R.rho) + cfR;
   const T a = aL > aR ? aL : aR;
   return hll_flux(L, R, -a, a, gm1);
+}
+
+// HLL with the Davis speeds of the HLLD solver below, after the same normal
+// field averaging (the flux hlld_flux falls back to on degenerate fans,
+// include/quasar/numerics/hlld_core.hpp:157-192, 241-290).  The cheap path
+// of the learned form and the HLLD-vs-HLL label use it.
+template <class T>
+__host__ __device__ __forceinline__ St<T> hll_davis_flux(const St<T>& Lin, const St<T>& Rin, T gamma) {
+  const T gm1 = gamma - T(1);
+  const T bn = T(0.5) * (Lin.bx + Rin.bx);
+  St<T> L = Lin, R = Rin;
+  const T hbn2 = T(0.5) * bn * bn;
+  L.en = (L.en - T(0.5) * L.bx * L.bx) + hbn2;
+  R.en = (R.en - T(0.5) * R.bx * R.bx) + hbn2;
+  L.bx = bn;
+  R.bx = bn;
+  const T vxL = L.mx / L.rho, vxR = R.mx / R.rho;
+  const T cfL = fast_speed_x(L.rho, pressure(L, gm1), L.bx, L.by, L.bz, gamma);
+  const T cfR = fast_speed_x(R.rho, pressure(R, gm1), R.bx, R.by, R.bz, gamma);
+  const T cfmax = cfL > cfR ? cfL : cfR;
+  const T sl = (vxL < vxR ? vxL : vxR) - cfmax;
+  const T sr = (vxL > vxR ? vxL : vxR) + cfmax;
+  return hll_flux(L, R, sl, sr, gm1);
 }
 
 template <class T>

@@ -25,6 +25,7 @@
 
 #include "../../include/quasar_mhd.h"
 #include "qmhd_kernels.hpp"
+#include "qmhd_mlp.hpp"
 
 namespace qmhd {
 namespace {
@@ -307,6 +308,14 @@ class Solver {
       fp32_ = true;
     else if (arithmetic != "fp64")
       throw std::invalid_argument("MhdSolver2D: arithmetic must be 'fp64' or 'fp32'");
+    const std::string indicator = cfg.indicator;
+    if (indicator == "mlp") {
+      if (order_ != 5 && order_ != 7)
+        throw std::invalid_argument("MhdSolver2D: the learned indicator needs reconstruction 'mp5' or 'mp7'");
+      learned_ = true;
+    } else if (!indicator.empty() && indicator != "none") {
+      throw std::invalid_argument("MhdSolver2D: indicator must be 'none' or 'mlp'");
+    }
     if (std::string(cfg.riemann) != "hlld")
       throw std::invalid_argument("MhdSolver2D: device evolution supports only riemann='hlld'");
     if (std::string(cfg.ct) != "fd_ct_christlieb")
@@ -348,6 +357,13 @@ class Solver {
     ez_ = DBuf(n);
     fb_list_ = DBuf(n);
     fb_count_ = DBuf(1);
+    if (learned_) {
+      mlp_params_ = DBuf((mlp::kParams + 1) / 2);
+      mlp_packed_ = DBuf((mlp::kPackTotal + 3) / 4);
+      cell_flags_ = DBuf((n + 7) / 8);
+      plain_total_ = DBuf(1);
+      QMHD_HIP(hipMemset(plain_total_.p, 0, sizeof(double)));
+    }
     QMHD_HIP(hipMalloc(&status_, sizeof(StepStatus)));
     QMHD_HIP(hipMemset(status_, 0, sizeof(StepStatus)));
     QMHD_HIP(hipHostMalloc(&host_status_, sizeof(StepStatus), hipHostMallocDefault));
@@ -469,6 +485,20 @@ class Solver {
       fyp[c] = fy_[c].p;
     }
     (void)want_rates;
+    if (learned_ && order == order_) {
+      if (!indicator_set_)
+        throw std::invalid_argument("MhdSolver2D: indicator 'mlp' needs its network (set_indicator) before stepping");
+      // Once per step, on the live state: within one step a feature moves by
+      // at most the CFL fraction of a cell, well inside the neighbourhood the
+      // flags cover (the face rule takes either neighbour's flag).
+      if (slot == L_)
+        clock_.timed(QMHD_KERNEL_INDICATOR, stream_, [&] {
+          launch_indicator(g_, gamma_, r.view(), r.bxc.p, r.byc.p, packed(), params(), threshold_, flags(),
+                           stream_);
+        });
+      faces_issued_ += static_cast<long long>(g_.nx + 1) * (g_.ny + 2 * g_.ng) +
+                       static_cast<long long>(g_.nx + 2 * g_.ng) * (g_.ny + 1);
+    }
     if (fp32_)
       issue_faces_as<float>(r, order, hll, ip, fxp, fyp);
     else
@@ -476,20 +506,27 @@ class Solver {
   }
 
   template <class T>
+  void issue_reconstruct(Register& r, int dir, int order, double* const* ip, int* eq) {
+    if (learned_ && order == order_)
+      launch_reconstruct_learned<T>(g_, dir, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, flags(),
+                                    iface_as<T>(ip), eq, fb_count(), fb_list(), plain_total(), stream_);
+    else
+      launch_reconstruct<T>(g_, dir, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface_as<T>(ip), eq,
+                            fb_count(), fb_list(), stream_);
+  }
+
+  template <class T>
   void issue_faces_as(Register& r, int order, int hll, double* const* ip, double* const* fxp, double* const* fyp) {
-    clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_, [&] {
-      launch_reconstruct<T>(g_, 0, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface_as<T>(ip),
-                            reinterpret_cast<int*>(eqx_.p), fb_count(), fb_list(), stream_);
-    });
+    const uint8_t* cf = learned_ && order == order_ ? flags() : nullptr;
+    clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_,
+                 [&] { issue_reconstruct<T>(r, 0, order, ip, reinterpret_cast<int*>(eqx_.p)); });
     clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
-      launch_flux<T>(g_, 0, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fxp), rate_x_.p, stream_);
+      launch_flux<T>(g_, 0, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fxp), rate_x_.p, stream_, cf);
     });
-    clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_, [&] {
-      launch_reconstruct<T>(g_, 1, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface_as<T>(ip),
-                            reinterpret_cast<int*>(eqy_.p), fb_count(), fb_list(), stream_);
-    });
+    clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_,
+                 [&] { issue_reconstruct<T>(r, 1, order, ip, reinterpret_cast<int*>(eqy_.p)); });
     clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
-      launch_flux<T>(g_, 1, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fyp), rate_y_.p, stream_);
+      launch_flux<T>(g_, 1, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fyp), rate_y_.p, stream_, cf);
     });
   }
 
@@ -497,8 +534,9 @@ class Solver {
   template <class T>
   void issue_ct_update(Register& in, Register& out, double a, double b, double c_host, const double* dt_dev,
                        const Register& us, int order, double* const* fxp, double* const* fyp) {
+    const uint8_t* cf = learned_ && order == order_ ? flags() : nullptr;
     clock_.timed(QMHD_KERNEL_CELL_EMF, stream_, [&] {
-      launch_cell_emf<T>(g_, order, field_, gamma_, in.view(), in.bxc.p, in.byc.p, cell_ez_.p, stream_);
+      launch_cell_emf<T>(g_, order, field_, gamma_, in.view(), in.bxc.p, in.byc.p, cell_ez_.p, stream_, cf);
     });
     clock_.timed(QMHD_KERNEL_CORNER, stream_, [&] {
       launch_corner<T>(g_, order, field_, flux_as<T>(fxp), flux_as<T>(fyp), cell_ez_.p,
@@ -592,6 +630,48 @@ class Solver {
     *launches = clock_.launches(kind);
   }
   void* stream() const { return stream_; }
+
+  // -- learned form ---------------------------------------------------------
+  void set_indicator(const float* host_params, float threshold) {
+    if (!learned_) throw std::invalid_argument("MhdSolver2D: set_indicator needs indicator 'mlp' in the config");
+    for (int k = 0; k < mlp::kParams; ++k)
+      if (!std::isfinite(host_params[k])) throw std::invalid_argument("MhdSolver2D: non-finite indicator parameter");
+    if (std::isnan(threshold)) throw std::invalid_argument("MhdSolver2D: indicator threshold is NaN");
+    QMHD_HIP(hipMemcpyAsync(params(), host_params, sizeof(float) * mlp::kParams, hipMemcpyHostToDevice, stream_));
+    launch_mlp_pack(params(), packed(), stream_);
+    QMHD_HIP(hipGetLastError());
+    QMHD_HIP(hipStreamSynchronize(stream_));
+    threshold_ = threshold;
+    indicator_set_ = true;
+  }
+  void plain_faces(long long* plain, long long* total) {
+    if (!learned_) {
+      *plain = *total = 0;
+      return;
+    }
+    unsigned long long v = 0;
+    QMHD_HIP(hipMemcpyAsync(&v, plain_total_.p, sizeof v, hipMemcpyDeviceToHost, stream_));
+    QMHD_HIP(hipMemsetAsync(plain_total_.p, 0, sizeof v, stream_));
+    QMHD_HIP(hipStreamSynchronize(stream_));
+    *plain = static_cast<long long>(v);
+    *total = faces_issued_;
+    faces_issued_ = 0;
+  }
+  void cell_features(float* dev_out) {
+    issue_live_prep();
+    launch_features(g_, gamma_, reg_[L_].view(), reg_[L_].bxc.p, reg_[L_].byc.p, dev_out, stream_);
+    QMHD_HIP(hipGetLastError());
+    QMHD_HIP(hipStreamSynchronize(stream_));
+  }
+  void cell_labels(double tol, uint8_t* dev_out) {
+    issue_live_prep();
+    uint8_t* fx = reinterpret_cast<uint8_t*>(cell_ez_.p);
+    uint8_t* fy = reinterpret_cast<uint8_t*>(ez_.p);
+    launch_labels(g_, order_, field_, gamma_, reg_[L_].view(), reg_[L_].bxc.p, reg_[L_].byc.p, tol, fx, fy, dev_out,
+                  stream_);
+    QMHD_HIP(hipGetLastError());
+    QMHD_HIP(hipStreamSynchronize(stream_));
+  }
 
  private:
 
@@ -737,6 +817,16 @@ class Solver {
   double gamma_{5.0 / 3.0}, cfl_{0.4};
   int order_{7};
   bool fp32_{false};  // FP32 face arithmetic (reconstruction, fluxes, cell E_z quadrature)
+  // Learned form: indicator network, its BF16 operands, per-cell flags and
+  // the running count of faces sent to the plain path.
+  bool learned_{false}, indicator_set_{false};
+  float threshold_{0.f};
+  long long faces_issued_{0};
+  DBuf mlp_params_, mlp_packed_, cell_flags_, plain_total_;
+  float* params() { return reinterpret_cast<float*>(mlp_params_.p); }
+  uint16_t* packed() { return reinterpret_cast<uint16_t*>(mlp_packed_.p); }
+  uint8_t* flags() { return reinterpret_cast<uint8_t*>(cell_flags_.p); }
+  unsigned long long* plain_total() { return reinterpret_cast<unsigned long long*>(plain_total_.p); }
   Flags4 fluid_{}, field_{};
   hipStream_t stream_{nullptr};
   Register reg_[4];
@@ -1204,8 +1294,8 @@ int qmhd_solver_kernel_time(qmhd_solver* s, int kind, double* total_ms, long lon
   });
 }
 const char* qmhd_kernel_kind_name(int kind) {
-  static const char* names[QMHD_KERNEL_KINDS] = {"prep", "reconstruct", "flux", "cell_emf",
-                                                 "corner", "update", "ghosts", "reduce"};
+  static const char* names[QMHD_KERNEL_KINDS] = {"prep",   "reconstruct", "flux",   "cell_emf", "corner",
+                                                 "update", "ghosts",      "reduce", "indicator"};
   return kind >= 0 && kind < QMHD_KERNEL_KINDS ? names[kind] : "";
 }
 void* qmhd_solver_stream(qmhd_solver* s) { return s->impl.stream(); }
@@ -1217,6 +1307,123 @@ int qmhd_solver_state_plane(qmhd_solver* s, int component, double** device_ptr) 
     if (component < 0 || component > 7) throw std::invalid_argument("component must be 0..7");
     *device_ptr = s->impl.plane(component);
   });
+}
+
+
+// ---- learned form -------------------------------------------------------
+
+int qmhd_mlp_pack(const float* params, uint16_t* packed, void* stream) {
+  return guarded([&] {
+    qmhd::launch_mlp_pack(params, packed, st(stream));
+    qmhd::hip_check(hipGetLastError(), "mlp pack");
+  });
+}
+
+int qmhd_mlp_forward(const uint16_t* packed, const float* params, const float* features, int64_t n, float* logits,
+                     void* stream) {
+  return guarded([&] {
+    if (n < 0) throw std::invalid_argument("qmhd_mlp_forward: negative sample count");
+    if (n == 0) return;
+    qmhd::launch_mlp_forward(packed, params, features, n, logits, st(stream));
+    qmhd::hip_check(hipGetLastError(), "mlp forward");
+  });
+}
+
+size_t qmhd_mlp_partial_floats(int64_t n, int chunks) {
+  if (n <= 0 || chunks <= 0) return 0;
+  const long long waves = qmhd::mlp_grad_waves(n, chunks);
+  return static_cast<size_t>((waves + 3) / 4 * 4) * (qmhd::mlp::kParams + 1);
+}
+
+int qmhd_mlp_gradient(const uint16_t* packed, const float* params, const float* features, const float* targets,
+                      const float* weights, const int64_t* order, int64_t n, int chunks, float* partial,
+                      float* grad_and_loss, void* stream) {
+  return guarded([&] {
+    if (n <= 0 || chunks <= 0) throw std::invalid_argument("qmhd_mlp_gradient: n and chunks must be positive");
+    qmhd::launch_mlp_grad(packed, params, features, targets, weights, reinterpret_cast<const long long*>(order), n,
+                          chunks, partial, grad_and_loss, st(stream));
+    qmhd::hip_check(hipGetLastError(), "mlp gradient");
+  });
+}
+
+int qmhd_mlp_adam(float* params, const float* grad, float* m, float* v, int64_t step, float lr, float beta1,
+                  float beta2, float eps, void* stream) {
+  return guarded([&] {
+    if (step < 1) throw std::invalid_argument("qmhd_mlp_adam: step counts from 1");
+    qmhd::launch_mlp_adam(params, grad, m, v, step, lr, beta1, beta2, eps, st(stream));
+    qmhd::hip_check(hipGetLastError(), "mlp adam");
+  });
+}
+
+int qmhd_cell_features(const qmhd_grid* g, double gamma, const double* const* u8, float* features, double* scratch,
+                       void* stream) {
+  return guarded([&] {
+    const qmhd::GridD d = qmhd::to_grid(*g);
+    if (d.ng < qmhd::mlp::kHalo) throw std::invalid_argument("qmhd_cell_features: nghost must be at least 2");
+    const Scratch s = scratch_of(g, scratch);
+    qmhd::ensure_quad_rows();
+    qmhd::launch_collocate(d, 0, 0, u8[5], s.plane(0), st(stream));
+    qmhd::launch_collocate(d, 1, 0, u8[6], s.plane(1), st(stream));
+    qmhd::launch_features(d, gamma, qmhd::view(u8), s.plane(0), s.plane(1), features, st(stream));
+    qmhd::hip_check(hipGetLastError(), "cell features");
+  });
+}
+
+int qmhd_cell_labels(const qmhd_grid* g, int order, const int32_t* flags4, double gamma, const double* const* u8,
+                     double label_tol, uint8_t* cell_bits, double* scratch, void* stream) {
+  return guarded([&] {
+    const qmhd::GridD d = qmhd::to_grid(*g);
+    if (order != 5 && order != 7) throw std::invalid_argument("qmhd_cell_labels: order must be 5 or 7");
+    if (d.ng < (order == 7 ? 4 : 3)) throw std::invalid_argument("qmhd_cell_labels: nghost too small for the order");
+    const Scratch s = scratch_of(g, scratch);
+    qmhd::ensure_quad_rows();
+    qmhd::launch_collocate(d, 0, 0, u8[5], s.plane(0), st(stream));
+    qmhd::launch_collocate(d, 1, 0, u8[6], s.plane(1), st(stream));
+    qmhd::launch_labels(d, order, qmhd::flags_of(flags4), gamma, qmhd::view(u8), s.plane(0), s.plane(1), label_tol,
+                        reinterpret_cast<uint8_t*>(s.plane(2)), reinterpret_cast<uint8_t*>(s.plane(3)), cell_bits,
+                        st(stream));
+    qmhd::hip_check(hipGetLastError(), "cell labels");
+  });
+}
+
+int qmhd_cell_indicator(const qmhd_grid* g, double gamma, const double* const* u8, const uint16_t* packed,
+                        const float* params, float threshold_logit, uint8_t* flags, double* scratch, void* stream) {
+  return guarded([&] {
+    const qmhd::GridD d = qmhd::to_grid(*g);
+    if (d.ng < qmhd::mlp::kHalo) throw std::invalid_argument("qmhd_cell_indicator: nghost must be at least 2");
+    const Scratch s = scratch_of(g, scratch);
+    qmhd::ensure_quad_rows();
+    qmhd::launch_collocate(d, 0, 0, u8[5], s.plane(0), st(stream));
+    qmhd::launch_collocate(d, 1, 0, u8[6], s.plane(1), st(stream));
+    qmhd::launch_indicator(d, gamma, qmhd::view(u8), s.plane(0), s.plane(1), packed, params, threshold_logit, flags,
+                           st(stream));
+    qmhd::hip_check(hipGetLastError(), "cell indicator");
+  });
+}
+
+int qmhd_stream_synchronize(void* stream) {
+  return guarded([&] { qmhd::hip_check(hipStreamSynchronize(st(stream)), "hipStreamSynchronize"); });
+}
+
+int qmhd_solver_set_indicator(qmhd_solver* s, const float* host_params, float threshold_logit) {
+  return guarded([&] { s->impl.set_indicator(host_params, threshold_logit); });
+}
+
+int qmhd_solver_plain_faces(qmhd_solver* s, int64_t* plain, int64_t* total) {
+  return guarded([&] {
+    long long p = 0, t = 0;
+    s->impl.plain_faces(&p, &t);
+    *plain = p;
+    *total = t;
+  });
+}
+
+int qmhd_solver_cell_features(qmhd_solver* s, float* features) {
+  return guarded([&] { s->impl.cell_features(features); });
+}
+
+int qmhd_solver_cell_labels(qmhd_solver* s, double label_tol, uint8_t* cell_bits) {
+  return guarded([&] { s->impl.cell_labels(label_tol, cell_bits); });
 }
 
 }  // extern "C"

@@ -414,12 +414,17 @@ __host__ __device__ __forceinline__ bool admissible_state(const St<T>& s, T gamm
 // Replace the normal field of a face state keeping its pressure: the energy
 // gains v^2/2 - bn^2/2 as one correctly rounded sum of the rounded products
 // (mhd_reconstruct.hip:141-153).
+// The FP32 form takes the plain FP32 sum.
 template <class T>
 __host__ __device__ __forceinline__ void set_normal_b(St<T>& s, int dir, T v) {
 #pragma clang fp contract(off)
   T& bn = dir == 0 ? s.bx : s.by;
-  const double t[3] = {double(s.en), double(v * (T(0.5) * v)), double(bn * (-T(0.5) * bn))};
-  s.en = T(cr_sum<3>(t));
+  if constexpr (sizeof(T) == 4) {
+    s.en = (s.en + v * (T(0.5) * v)) - bn * (T(0.5) * bn);
+  } else {
+    const double t[3] = {double(s.en), double(v * (T(0.5) * v)), double(bn * (-T(0.5) * bn))};
+    s.en = T(cr_sum<3>(t));
+  }
   bn = v;
 }
 
@@ -466,9 +471,12 @@ __host__ __device__ __noinline__ void muscl_face(const GridD& g, int dir, int i,
   ur = to_cons(face(w[1], w[2], w[3], T(-1), drop_p), gamma);
 }
 
+// `clip_rel` (optional) receives the largest change any of the 14 limiter
+// calls made, relative to the largest component of the reference state.
 template <class T, int ORDER>
 __host__ __device__ __forceinline__ void mp_char_face(const GridD& g, int dir, int i, int j, T gamma, const StateView& u,
-                                             const double* bxc, const double* byc, St<T>& ul, St<T>& ur) {
+                                             const double* bxc, const double* byc, St<T>& ul, St<T>& ur,
+                                             T* clip_rel = nullptr) {
   constexpr int H = ORDER == 5 ? 2 : 3;
   constexpr int UW = 2 * H + 2;
   constexpr int W = 2 * H + 1;
@@ -495,6 +503,8 @@ __host__ __device__ __forceinline__ void mp_char_face(const GridD& g, int dir, i
     basis_to_char(B, d, wch[q]);
   }
   T wl[7], wr[7];
+  T shift = T(0);
+  T* const sp = clip_rel ? &shift : nullptr;
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
     // Left state: cells 0..W-1 (centre H), right state: reversed cells 1..W.
@@ -507,8 +517,14 @@ __host__ __device__ __forceinline__ void mp_char_face(const GridD& g, int dir, i
     const T fl = ORDER == 5 ? mp5_face(sl) : mp7_face(sl);
     const T fr = ORDER == 5 ? mp5_face(sr) : mp7_face(sr);
     constexpr int o = ORDER == 5 ? 0 : 1;
-    wl[k] = mp_limit(fl, sl[o], sl[o + 1], sl[o + 2], sl[o + 3], sl[o + 4]);
-    wr[k] = mp_limit(fr, sr[o], sr[o + 1], sr[o + 2], sr[o + 3], sr[o + 4]);
+    wl[k] = mp_limit(fl, sl[o], sl[o + 1], sl[o + 2], sl[o + 3], sl[o + 4], sp);
+    wr[k] = mp_limit(fr, sr[o], sr[o + 1], sr[o + 2], sr[o + 3], sr[o + 4], sp);
+  }
+  if (clip_rel) {
+    T scale = T(0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) scale = fmax(scale, qabs(comp(ref, c)));
+    *clip_rel = shift / (scale + Lim<T>::tiny);
   }
   T aL[7], aR[7];
   basis_from_char(B, wl, aL);
@@ -519,14 +535,26 @@ __host__ __device__ __forceinline__ void mp_char_face(const GridD& g, int dir, i
   ur = rot(sR, dir);
 }
 
+// Path bits of a reconstructed face (the troubled-cell labels): an MP limiter
+// changed a face value by more than `tol` of the state's scale, the MUSCL
+// fallback or the piecewise-constant fallback (or NaN sentinel) was taken,
+// the HLLD and HLL fluxes differ by more than `tol` of the flux scale.
+constexpr unsigned kPathMpClip = 1u, kPathMuscl = 2u, kPathConstant = 4u, kPathHllGap = 8u;
+struct LabelProbe {
+  unsigned bits;
+  double tol;
+};
+
 // Interface states of one face normal to `dir` at the scheme's order with the
 // reference's fallback chain (mhd_reconstruct.hip:465-670): characteristic MP
 // -> MUSCL when an MP state is inadmissible; after the normal field is set,
-// piecewise-constant cell states, then a NaN sentinel.
+// piecewise-constant cell states, then a NaN sentinel.  `probe` (optional)
+// receives the path bits above.
 template <class T, int ORDER>
 __host__ __device__ __forceinline__ void reconstruct_face(const GridD& g, int dir, int i, int j, T gamma, int side_lo,
                                                           int side_hi, const StateView& u, const double* bxc,
-                                                          const double* byc, St<T>& ul, St<T>& ur) {
+                                                          const double* byc, St<T>& ul, St<T>& ur,
+                                                          LabelProbe* probe = nullptr) {
   const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
   if (ORDER <= 1) {
     ul = load_cell_lo<T>(g, u, i - di, j - dj);
@@ -534,14 +562,19 @@ __host__ __device__ __forceinline__ void reconstruct_face(const GridD& g, int di
   } else if (ORDER == 2) {
     muscl_face<T>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
   } else {
-    mp_char_face<T, ORDER>(g, dir, i, j, gamma, u, bxc, byc, ul, ur);
-    if (!admissible_state(ul, gamma) || !admissible_state(ur, gamma))
+    T clip = T(0);
+    mp_char_face<T, ORDER>(g, dir, i, j, gamma, u, bxc, byc, ul, ur, probe ? &clip : nullptr);
+    if (probe && double(clip) > probe->tol) probe->bits |= kPathMpClip;
+    if (!admissible_state(ul, gamma) || !admissible_state(ur, gamma)) {
+      if (probe) probe->bits |= kPathMuscl;
       muscl_face<T>(g, dir, i, j, gamma, side_lo, side_hi, u, bxc, byc, ul, ur);
+    }
   }
   const T bn = T((dir == 0 ? u.c[5] : u.c[6])[idx(g, i, j)]);
   set_normal_b(ul, dir, bn);
   set_normal_b(ur, dir, bn);
   if (!admissible_state(ul, gamma) || !admissible_state(ur, gamma)) {
+    if (probe) probe->bits |= kPathConstant;
     if (ORDER <= 1) {
       ul = load_cell_lo<T>(g, u, i - di, j - dj);
       ur = load_cell_lo<T>(g, u, i, j);

@@ -186,14 +186,18 @@ class MhdConfig:
         self.timestep_signature = ""
         self.boundary = MhdBoundarySpec()
         self.background = MhdBackgroundSpec()
-        # Extension: "fp64" (the reference's arithmetic) or "fp32" face work.
+        # Extensions: "fp64" (the reference's arithmetic) or "fp32" face work;
+        # troubled-cell indicator "none" (the plain method) or "mlp" (the
+        # learned form, network set with MhdSolver2D.set_indicator).
         self.arithmetic = "fp64"
+        self.indicator = "none"
 
     def _c(self) -> _lib.QSolverConfig:
         c = _lib.QSolverConfig()
         c.grid = self.grid._c()
         c.gamma, c.cfl, c.rho_floor, c.p_floor = self.gamma, self.cfl, self.rho_floor, self.p_floor
-        for name in ("reconstruction", "riemann", "integrator", "ct", "positivity", "geometry", "arithmetic"):
+        for name in ("reconstruction", "riemann", "integrator", "ct", "positivity", "geometry", "arithmetic",
+                     "indicator"):
             setattr(c, name, getattr(self, name).encode())
         for s in range(4):
             c.fluid[s].value = self.boundary.fluid[s].encode()
@@ -339,6 +343,30 @@ class MhdSolver2D:
         _lib.check(self._lib.qmhd_solver_run(self._h, int(steps), float(t_end), C.byref(done),
                                              C.byref(t)))
         return done.value, t.value
+
+    # -- learned form -------------------------------------------------------
+    def set_indicator(self, params, threshold_logit: float) -> None:
+        """Network (FP32 vector of MLP_PARAMS) and logit threshold of the
+        troubled-cell indicator; needs indicator "mlp" in the config."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.shape != (_lib.MLP_PARAMS,):
+            raise ValueError(f"indicator parameters must have shape ({_lib.MLP_PARAMS},)")
+        _lib.check(self._lib.qmhd_solver_set_indicator(self._h, p.ctypes.data_as(C.c_void_p),
+                                                       float(threshold_logit)))
+
+    def plain_faces(self):
+        """(faces sent to the plain path, faces reconstructed) since the last call."""
+        a, b = C.c_int64(), C.c_int64()
+        _lib.check(self._lib.qmhd_solver_plain_faces(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def cell_features_into(self, device_ptr: int) -> None:
+        """Features of the interior cells (nx*ny x 100 FP32) into device memory."""
+        _lib.check(self._lib.qmhd_solver_cell_features(self._h, C.c_void_p(device_ptr)))
+
+    def cell_labels_into(self, label_tol: float, device_ptr: int) -> None:
+        """Label bits of the interior cells (nx*ny bytes) into device memory."""
+        _lib.check(self._lib.qmhd_solver_cell_labels(self._h, float(label_tol), C.c_void_p(device_ptr)))
 
     def set_profiling(self, enable: bool) -> None:
         """Bracket every launch with HIP events on the solver's stream (resets

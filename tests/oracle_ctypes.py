@@ -111,6 +111,14 @@ class Oracle:
         L.orc_solver_last_substeps.argtypes = [C.c_void_p]
         L.orc_solver_counters.argtypes = [C.c_void_p, C.POINTER(OrcCounters)]
         L.orc_last_error.restype = C.c_char_p
+        FP = C.POINTER(C.c_float)
+        U8P = C.POINTER(C.c_uint8)
+        L.orc_cell_features.argtypes = [C.POINTER(OrcGrid), D, C.c_void_p, FP]
+        L.orc_cell_labels.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_void_p, D, C.c_void_p, D, U8P]
+        L.orc_hll_davis_flux_x.argtypes = [DP, DP, D, DP]
+        L.orc_mlp_forward.argtypes = [FP, FP, C.c_longlong, C.c_int, DP]
+        L.orc_mlp_loss_grad.argtypes = [FP, DP, FP, FP, FP, C.c_longlong, C.c_int, DP]
+        L.orc_mlp_loss_grad.restype = D
 
     # -- scalar numerics --------------------------------------------------
     def cr_sum(self, terms):
@@ -273,6 +281,47 @@ class Oracle:
 
     def last_error(self):
         return self.lib.orc_last_error().decode()
+
+    # -- learned form -----------------------------------------------------
+    def cell_features(self, g, gamma, u8):
+        out = np.zeros((g.nx * g.ny, 100), dtype=np.float32)
+        self.lib.orc_cell_features(C.byref(g), D(gamma), _planes(u8),
+                                   out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def cell_labels(self, g, order, flags, gamma, u8, label_tol):
+        out = np.zeros(g.nx * g.ny, dtype=np.uint8)
+        self.lib.orc_cell_labels(C.byref(g), order, _flags(flags), D(gamma), _planes(u8), D(label_tol),
+                                 out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return out
+
+    def hll_davis_flux_x(self, l, r, gamma):
+        f = np.zeros(8)
+        self.lib.orc_hll_davis_flux_x(_ptr(np.asarray(l, float)), _ptr(np.asarray(r, float)), gamma, _ptr(f))
+        return f
+
+    def mlp_forward(self, params, x, bf16=True):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        out = np.zeros(x.shape[0])
+        self.lib.orc_mlp_forward(p.ctypes.data_as(C.POINTER(C.c_float)), x.ctypes.data_as(C.POINTER(C.c_float)),
+                                 x.shape[0], 1 if bf16 else 0, _ptr(out))
+        return out
+
+    def mlp_loss_grad(self, params, x, y, w, bf16=True, params64=None):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        p64 = None if params64 is None else np.ascontiguousarray(params64, dtype=np.float64)
+        grad = np.zeros(p.shape[0])
+        loss = self.lib.orc_mlp_loss_grad(p.ctypes.data_as(C.POINTER(C.c_float)),
+                                          None if p64 is None else _ptr(p64),
+                                          x.ctypes.data_as(C.POINTER(C.c_float)),
+                                          y.ctypes.data_as(C.POINTER(C.c_float)),
+                                          w.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0],
+                                          1 if bf16 else 0, _ptr(grad))
+        return loss, grad
 
 
 def ic_spec(kind: str, gamma: float, **kw) -> OrcIcSpec:
