@@ -77,7 +77,9 @@ def parse():
                     help="bound on the CPU baseline's timed work")
     ap.add_argument("--cpu-n", type=int, default=256, help="cells per side of the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form run")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP32 and FP32+learned runs")
+    ap.add_argument("--noise", type=float, default=0.0,
+                    help="seeded white-noise amplitude on rho (relative), vx and vy (the 4096^2 deck uses 1e-3)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the timed region, copy a buffer of this size once (HBM traffic calibration "
                          "for counter runs)")
@@ -88,7 +90,7 @@ def parse():
     return ap.parse_args()
 
 
-def orszag_tang(n: int):
+def orszag_tang(n: int, noise: float = 0.0):
     from quasar_amd import mhd
     spec = mhd.MhdInitialConditionSpec()
     spec.kind = "orszag_tang"
@@ -97,17 +99,20 @@ def orszag_tang(n: int):
     spec.b_uniform = [1.0 / math.sqrt(4.0 * math.pi), 0.0, 0.0]
     spec.ot_rho = 25.0 / (36.0 * math.pi)
     spec.ot_p = 5.0 / (12.0 * math.pi)
+    spec.noise_amplitude = noise
+    spec.noise_seed = 0
     return spec
 
 
-def make_solver(n: int, arithmetic: str = "fp64"):
+def make_solver(n: int, arithmetic: str = "fp64", indicator: str = "none", noise: float = 0.0):
     from quasar_amd import mhd
-    spec = orszag_tang(n)
+    spec = orszag_tang(n, noise)
     cfg = mhd.MhdConfig()
     cfg.grid = spec.grid
     cfg.gamma = spec.gamma
     cfg.reconstruction = "mp7"
     cfg.arithmetic = arithmetic
+    cfg.indicator = indicator
     cfg.cfl = 0.4
     cfg.boundary.set_fluid_all("periodic")
     cfg.boundary.set_field_all("periodic")
@@ -115,6 +120,10 @@ def make_solver(n: int, arithmetic: str = "fp64"):
     state = mhd.build_initial_state(spec)
     for name, plane in state.items():
         solver.seed_state(name, plane)
+    if indicator == "mlp":
+        from quasar_amd import learned
+        params, threshold, _ = learned.load()
+        solver.set_indicator(params, threshold)
     return solver
 
 
@@ -313,7 +322,7 @@ def main():
 
     n = args.n
     print(f"bench: seeding Orszag-Tang {n}^2 MP7 FP64", file=sys.stderr, flush=True)
-    solver = make_solver(n)
+    solver = make_solver(n, noise=args.noise)
     if args.copy_probe_mib > 0:
         a = torch.ones(args.copy_probe_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
@@ -346,31 +355,42 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: Orszag-Tang initial condition from its closed form",
+        "data": "synthetic: Orszag-Tang initial condition from its closed form" +
+                (f" plus seeded white noise of amplitude {args.noise:g} on rho, vx, vy" if args.noise else ""),
         "config": {"workload": f"orszag_tang_{n}x{n}_mp7_hlld_ct_ssprk3_fp64", "form": "plain FP64",
-                   "cfl": 0.4, "boundaries": "periodic", "sim_time": sim_t},
+                   "cfl": 0.4, "boundaries": "periodic", "sim_time": sim_t, "noise": args.noise},
         "hbm_frac_step": round(value * 512 / (PEAK_HBM_GBS * 1e9), 5),
         "kernel_ms": shares,
         "roofline": roof,
     }
-    if not args.no_fp32:
-        # The FP32 form on the same workload and step count, and its distance
-        # from the FP64 run after the same number of steps.
-        print("bench: FP32 form", file=sys.stderr, flush=True)
+    if not args.no_forms:
+        # The FP32 and FP32+learned forms on the same workload and step count,
+        # with their per-variable distance from the FP64 run after the same
+        # number of steps.
         ref_state = interior_state(solver, n)
         del solver
-        f32 = make_solver(n, "fp32")
-        if args.warmup > 0:
-            f32.run(args.warmup)
-        wall32, sim32 = timed_run(f32, args.steps)
-        line["forms"] = {"fp32": {
-            "value": round(n * n * args.steps / wall32, 1),
-            "ms_per_step": round(wall32 / args.steps * 1e3, 4),
-            "speedup_vs_fp64": round(wall / wall32, 3),
-            "sim_time": sim32,
-            "rel_l1_vs_fp64": rel_l1_by_variable(interior_state(f32, n), ref_state),
-        }}
-        del f32
+        forms = {}
+        for name, arith, ind in (("fp32", "fp32", "none"), ("fp32_learned", "fp32", "mlp")):
+            print(f"bench: {name} form", file=sys.stderr, flush=True)
+            s = make_solver(n, arith, ind, args.noise)
+            if args.warmup > 0:
+                s.run(args.warmup)
+            if ind == "mlp":
+                s.plain_faces()
+            w, st = timed_run(s, args.steps)
+            entry = {
+                "value": round(n * n * args.steps / w, 1),
+                "ms_per_step": round(w / args.steps * 1e3, 4),
+                "speedup_vs_fp64": round(wall / w, 3),
+                "sim_time": st,
+                "rel_l1_vs_fp64": rel_l1_by_variable(interior_state(s, n), ref_state),
+            }
+            if ind == "mlp":
+                plain, total = s.plain_faces()
+                entry["plain_face_share"] = round(plain / max(total, 1), 5)
+            forms[name] = entry
+            del s
+        line["forms"] = forms
     if not args.no_cpu_baseline:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_seconds)

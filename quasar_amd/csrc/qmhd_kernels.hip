@@ -362,13 +362,18 @@ __global__ void counter_reset_kernel(unsigned long long* __restrict__ c) {
   if (threadIdx.x == 0) c[0] = 0ull;
 }
 
-// Learned form, one thread per face (the reconstruction launch domain): a
-// face whose two cells the indicator left unflagged takes the linear
-// fifth-order upwind-biased face values (the unlimited MP5 interpolant,
-// include/quasar/numerics/mp_limiter.hpp:182-191) of the conserved cell
-// averages, component by component, and the face normal field.  Flagged
-// faces, and cheap states that are not admissible, are listed for the plain
-// path (characteristic MP with its fallback chain).
+// Learned form, one thread per face: a face whose two cells the indicator
+// left unflagged takes the linear fifth-order upwind-biased face values (the
+// unlimited MP5 interpolant, include/quasar/numerics/mp_limiter.hpp:182-191)
+// of the conserved cell averages, component by component, and the face normal
+// field.  Flagged faces, and cheap states that are not admissible, are listed
+// for the plain path (characteristic MP with its fallback chain).  The block
+// (64 x 4 faces) stages the cells its stencils read in LDS: 69 x 4 for x faces,
+// 64 x 9 for y faces.
+constexpr int kCheapTx = 64, kCheapTy = 4;
+constexpr int kCheapCells = (kCheapTx + 5) * kCheapTy > kCheapTx * (kCheapTy + 5) ? (kCheapTx + 5) * kCheapTy
+                                                                                    : kCheapTx * (kCheapTy + 5);
+
 template <class T>
 __global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir, T gamma, StateView u,
                                                                 const double* __restrict__ bxc,
@@ -377,22 +382,48 @@ __global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir
                                                                 IfaceView<T> out, int* __restrict__ eq_flag,
                                                                 unsigned long long* __restrict__ fb_count,
                                                                 unsigned long long* __restrict__ fb_list) {
+  __shared__ T cells[8][kCheapCells];
+  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
+  // Tile of cells: columns [ci0, ci0 + cw) x rows [cj0, cj0 + ch).
+  const int cw = kCheapTx + 5 * di, ch = kCheapTy + 5 * dj;
+  const int fi0 = static_cast<int>(blockIdx.x) * kCheapTx - (dir == 1 ? g.ng : 0);
+  const int fj0 = static_cast<int>(blockIdx.y) * kCheapTy - (dir == 0 ? g.ng : 0);
+  const int ci0 = fi0 - 3 * di, cj0 = fj0 - 3 * dj;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int t = tid; t < cw * ch; t += 256) {
+    const int a = t % cw, b = t / cw;
+    const int ii = ci0 + a, jj = cj0 + b;
+    const bool inside = ii >= -g.ng && ii < g.nx + g.ng && jj >= -g.ng && jj < g.ny + g.ng;
+    const std::size_t k = inside ? idx(g, ii, jj) : 0;
+    cells[0][t] = inside ? T(u.c[0][k]) : T(1);
+    cells[1][t] = inside ? T(u.c[1][k]) : T(0);
+    cells[2][t] = inside ? T(u.c[2][k]) : T(0);
+    cells[3][t] = inside ? T(u.c[3][k]) : T(0);
+    cells[4][t] = inside ? T(u.c[4][k]) : T(1);
+    cells[5][t] = inside ? T(bxc[k]) : T(0);
+    cells[6][t] = inside ? T(byc[k]) : T(0);
+    cells[7][t] = inside ? T(u.c[7][k]) : T(0);
+  }
+  __syncthreads();
   int i, j;
   const bool valid = recon_face_of(g, dir, i, j);
-  const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
   const std::size_t k = valid ? idx(g, i, j) : 0;
   St<T> ul, ur;
   bool need = false;
   if (valid) {
     need = (cell_flags[k] | cell_flags[idx(g, i - di, j - dj)]) != 0;
     if (!need) {
-      St<T> c[6];
-#pragma unroll
-      for (int q = 0; q < 6; ++q) c[q] = load_cell<T>(g, u, bxc, byc, i + (q - 3) * di, j + (q - 3) * dj);
+      // Cell q (0..5) of the stencil is tile cell (tx + 3 - 3 + q) along the
+      // normal: cells i-3 .. i+2.
+      const int base = dir == 0 ? threadIdx.y * cw + threadIdx.x : threadIdx.y * cw + threadIdx.x;
+      const int step = dir == 0 ? 1 : cw;
 #pragma unroll
       for (int cc = 0; cc < 8; ++cc) {
-        const T vl[5] = {comp(c[0], cc), comp(c[1], cc), comp(c[2], cc), comp(c[3], cc), comp(c[4], cc)};
-        const T vr[5] = {comp(c[5], cc), comp(c[4], cc), comp(c[3], cc), comp(c[2], cc), comp(c[1], cc)};
+        T v[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[q] = cells[cc][base + q * step];
+        const T vl[5] = {v[0], v[1], v[2], v[3], v[4]};
+        const T vr[5] = {v[5], v[4], v[3], v[2], v[1]};
         set_comp(ul, cc, mp5_face(vl));
         set_comp(ur, cc, mp5_face(vr));
       }
@@ -406,10 +437,10 @@ __global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir
   if (mask != 0ull) {
     const int lane = static_cast<int>(__lane_id());
     const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(fb_count, static_cast<unsigned long long>(__popcll(mask)));
-    base = __shfl(base, leader, 64);
-    if (need) fb_list[base + __popcll(mask & ((1ull << lane) - 1ull))] = k;
+    unsigned long long fb_base = 0;
+    if (lane == leader) fb_base = atomicAdd(fb_count, static_cast<unsigned long long>(__popcll(mask)));
+    fb_base = __shfl(fb_base, leader, 64);
+    if (need) fb_list[fb_base + __popcll(mask & ((1ull << lane) - 1ull))] = k;
   }
   if (!valid || need) return;
   write_faces(out, eq_flag, k, ul, ur);
@@ -613,7 +644,7 @@ constexpr int kEmfBx = 16;
 constexpr int kEmfBy = 16;
 
 // With LEARNED, a block none of whose cells the indicator flagged takes the
-// cell-average E_z (second order) for every cell.
+// cell-average E_z (second order) for every cell, without staging its tile.
 template <class T, int ORDER, bool LEARNED = false>
 __global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high, int phys_y_high, T gamma,
                                                        StateView u, const double* __restrict__ bxc,
@@ -630,6 +661,17 @@ __global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high,
   __shared__ T xrec[8][TY][kEmfBx];
   const int i0 = blockIdx.x * kEmfBx, j0 = blockIdx.y * kEmfBy;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  if (LEARNED) {
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    const bool active = i <= g.nx && j <= g.ny && !(i == g.nx && !phys_x_high) && !(j == g.ny && !phys_y_high);
+    if (!__syncthreads_or(active && cell_flags[idx(g, i, j)] != 0)) {
+      if (active) {
+        const std::size_t k = idx(g, i, j);
+        cell_ez[k] = static_cast<double>(physical_ez(T(u.c[0][k]), T(u.c[1][k]), T(u.c[2][k]), T(bxc[k]), T(byc[k])));
+      }
+      return;
+    }
+  }
   for (int t = tid; t < TX * TY; t += blockDim.x * blockDim.y) {
     const int a = t % TX, b = t / TX;
     const int ii = i0 + a - H, jj = j0 + b - H;
@@ -647,14 +689,7 @@ __global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high,
   __syncthreads();
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
   const bool active = i <= g.nx && j <= g.ny && !(i == g.nx && !phys_x_high) && !(j == g.ny && !phys_y_high);
-  if (LEARNED && !__syncthreads_or(active && cell_flags[idx(g, i, j)] != 0)) {
-    if (active) {
-      const int a0 = threadIdx.x + H, b0 = threadIdx.y + H;
-      cell_ez[idx(g, i, j)] = static_cast<double>(
-          physical_ez(tile[0][b0][a0], tile[1][b0][a0], tile[2][b0][a0], tile[5][b0][a0], tile[6][b0][a0]));
-    }
-    return;
-  }
+
   // Stencil bounds inside the padded domain (always true for the computed
   // cells when ng >= H + 1).
   const bool in_range = i - H >= -g.ng && i + H <= g.nx + g.ng - 1 && j - H >= -g.ng && j + H <= g.ny + g.ng - 1;
@@ -1300,7 +1335,6 @@ void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma
                                 const StateView& u, const double* bxc, const double* byc, const uint8_t* cell_flags,
                                 const IfaceView<T>& out, int* eq, unsigned long long* fb_count,
                                 unsigned long long* fb_list, unsigned long long* plain_total, hipStream_t stream) {
-  const dim3 b(64, 4);
   const int lx = dir == 0 ? g.nx + 1 : g.nx + 2 * g.ng;
   const int ly = dir == 0 ? g.ny + 2 * g.ng : g.ny + 1;
   const int slo = dir == 0 ? flags.side[0] : flags.side[2];
@@ -1310,7 +1344,8 @@ void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma
   // the device at the MP kernel's occupancy.
   const dim3 list_grid(1024), list_block(256);
   hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb_count);
-  hipLaunchKernelGGL((cheap_reconstruct_kernel<T>), blocks2d(lx, ly, b), b, 0, stream, g, dir, gamma_t, u, bxc, byc,
+  const dim3 cb(kCheapTx, kCheapTy);
+  hipLaunchKernelGGL((cheap_reconstruct_kernel<T>), blocks2d(lx, ly, cb), cb, 0, stream, g, dir, gamma_t, u, bxc, byc,
                      cell_flags, out, eq, fb_count, fb_list);
   if (order == 5)
     hipLaunchKernelGGL((reconstruct_fallback_kernel<T, 5>), list_grid, list_block, 0, stream, g, dir, gamma_t, slo, shi,
