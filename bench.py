@@ -78,6 +78,12 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=256, help="cells per side of the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP32 and FP32+learned runs")
+    ap.add_argument("--sweep", action="store_true",
+                    help="instead of the single-run benchmark: the batched sweep of 64 mixed-size runs "
+                         "(FP32 + learned, one graph launch per step, autotuned launch shapes)")
+    ap.add_argument("--sweep-runs", type=int, default=64)
+    ap.add_argument("--sweep-seq-steps", type=int, default=30,
+                    help="steps per run of the run-by-run comparison of the sweep")
     ap.add_argument("--noise", type=float, default=0.0,
                     help="seeded white-noise amplitude on rho (relative), vx and vy (the 4096^2 deck uses 1e-3)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -160,6 +166,78 @@ def timed_run(solver, steps: int):
     if done != steps:
         raise RuntimeError(f"run stopped after {done} of {steps} steps")
     return wall, sim_t
+
+
+def sweep(args) -> dict:
+    """Batched sweep: FP32 + learned runs advanced together by one graph launch
+    per step, against the same runs stepped one after another and against the
+    FP64 plain form (batched) for the per-run error."""
+    import numpy as np
+    from quasar_amd import batch
+    descs = batch.sweep_runs(args.sweep_runs, seed=0)
+    cells = sum(d["n"] ** 2 for d in descs)
+    print(f"bench: sweep of {len(descs)} runs, {cells} cells, building FP32+learned runs (autotuned)",
+          file=sys.stderr, flush=True)
+    runs, tuned = batch.build_sweep(descs, "fp32", "mlp", autotune_steps=2)
+    b = batch.Batch(runs)
+    b.run(args.warmup)
+    t0 = time.perf_counter()
+    b.run(args.steps)
+    wall = time.perf_counter() - t0
+    status = [b.status(k) for k in range(len(descs))]
+    rejected = sum(st["rejected"] for st in status)
+    fatal = sum(1 for st in status if st["fatal"])
+    accepted = sum(d["n"] ** 2 * st["accepted"] for d, st in zip(descs, status))
+    value = cells * args.steps / wall
+    print(f"bench: batched {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+    # Run-by-run comparison (each run's own loop, one after another).
+    seq_runs, _ = batch.build_sweep(descs, "fp32", "mlp")
+    for s_, d in zip(seq_runs, descs):
+        s_.set_tuning(tuned[d["n"]])
+        s_.run(1)
+    t0 = time.perf_counter()
+    for s_ in seq_runs:
+        s_.run(args.sweep_seq_steps)
+    for s_ in seq_runs:
+        s_.synchronize()
+    seq_wall = time.perf_counter() - t0
+    seq_value = cells * args.sweep_seq_steps / seq_wall
+    del seq_runs
+    print(f"bench: run-by-run {args.sweep_seq_steps} steps in {seq_wall:.3f} s", file=sys.stderr, flush=True)
+    # FP64 plain reference of every run (batched), same number of steps.
+    ref_runs, _ = batch.build_sweep(descs, "fp64", "none")
+    rb = batch.Batch(ref_runs)
+    rb.run(args.warmup)
+    t0 = time.perf_counter()
+    rb.run(args.steps)
+    ref_wall = time.perf_counter() - t0
+    l1 = []
+    for s_, r_, d in zip(runs, ref_runs, descs):
+        n = d["n"]
+        got, exp = _interior(s_, n), _interior(r_, n)
+        l1.append(rel_l1_by_variable(got, exp))
+    rho = np.array([e["rho"] for e in l1])
+    worst = np.array([max(e.values()) for e in l1])
+    return {
+        "metric": "zone-updates/s, batched sweep of independent runs on 1 MI355X (FP32 + learned)",
+        "value": round(value, 1), "unit": "zone-updates/s", "n_gpus": 1, "runs": len(descs), "cells": cells,
+        "steps": args.steps, "warmup": args.warmup, "wall_s": round(wall, 4), "accepted_zone_updates": accepted,
+        "rejected_steps": rejected, "fatal_runs": fatal,
+        "run_by_run_value": round(seq_value, 1), "batching_gain": round(value / seq_value, 3),
+        "fp64_batched_value": round(cells * args.steps / ref_wall, 1),
+        "speedup_vs_fp64_batched": round(ref_wall / wall, 3),
+        "rel_l1_vs_fp64": {"rho_median": float(np.median(rho)), "rho_max": float(rho.max()),
+                           "any_variable_max": float(worst.max())},
+        "sizes": {str(n): sum(1 for d in descs if d["n"] == n) for n in sorted({d["n"] for d in descs})},
+        "tuning": {str(k): v for k, v in tuned.items()},
+        "data": "synthetic: seeded Orszag-Tang and rotor initial conditions with 1e-3 white noise",
+    }
+
+
+def _interior(solver, n: int):
+    import numpy as np
+    return [np.asarray(solver.state_component_to_host(nm)).reshape(n + 2 * NG, n + 2 * NG)[NG:NG + n, NG:NG + n]
+            for nm in STATE_NAMES]
 
 
 def units(kind: str, n: int) -> int:
@@ -320,6 +398,9 @@ def main():
     from quasar_amd import _lib
     _lib.load()  # raises when the gfx950 library is missing: there is no CPU path
 
+    if args.sweep:
+        print(json.dumps(sweep(args)), flush=True)
+        return
     n = args.n
     print(f"bench: seeding Orszag-Tang {n}^2 MP7 FP64", file=sys.stderr, flush=True)
     solver = make_solver(n, noise=args.noise)

@@ -275,6 +275,30 @@ int qmhd_stream_synchronize(void* stream);
 int qmhd_solver_set_indicator(qmhd_solver* s, const float* host_params, float threshold_logit);
 /* Faces sent to the plain path and faces reconstructed since the last call. */
 int qmhd_solver_plain_faces(qmhd_solver* s, int64_t* plain, int64_t* total);
+/* ---- launch tuning and batched runs (no reference counterpart) --------
+ * Tuning vector (8 x int32): flux block x, y; corner block x, y; update
+ * block x, y (64..256 threads, multiples of 64); blocks of the persistent
+ * list launches of the learned form's plain path; reserved.
+ * qmhd_solver_autotune times every candidate over `trial_steps` steps (the
+ * live state is restored afterwards) and keeps the fastest. */
+int qmhd_solver_autotune(qmhd_solver* s, int trial_steps, int32_t* tuning_out);
+int qmhd_solver_set_tuning(qmhd_solver* s, const int32_t* tuning);
+int qmhd_solver_get_tuning(const qmhd_solver* s, int32_t* tuning);
+
+/* Many independent runs stepped together: the auto-dt step of every run
+ * (each on its own stream, with the accept/reject decision and commit on the
+ * device) is captured into a HIP graph, and one graph launch advances every
+ * run by one step.  A rejected step (an inadmissible stage) leaves that run's
+ * state unchanged and is counted; a run whose live state is invalid stops and
+ * is flagged fatal.  The runs stay owned by the caller and must be stepped only
+ * through the batch while it exists (readbacks are fine). */
+typedef struct qmhd_batch qmhd_batch;
+int qmhd_batch_create(qmhd_solver* const* runs, int count, qmhd_batch** out);
+void qmhd_batch_destroy(qmhd_batch* b);
+int qmhd_batch_run(qmhd_batch* b, int steps);
+int qmhd_batch_run_status(qmhd_batch* b, int run, int64_t* accepted, int64_t* rejected, int* fatal,
+                          double* sim_time);
+
 /* Features / label bits of the live state (device outputs, as above). */
 int qmhd_solver_cell_features(qmhd_solver* s, float* features);
 int qmhd_solver_cell_labels(qmhd_solver* s, double label_tol, uint8_t* cell_bits);

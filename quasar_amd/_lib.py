@@ -61,7 +61,8 @@ EXPORTS = [
     "qmhd_mlp_pack", "qmhd_mlp_forward", "qmhd_mlp_partial_floats", "qmhd_mlp_gradient", "qmhd_mlp_adam",
     "qmhd_cell_features", "qmhd_cell_labels", "qmhd_cell_indicator", "qmhd_stream_synchronize",
     "qmhd_solver_set_indicator", "qmhd_solver_plain_faces", "qmhd_solver_cell_features",
-    "qmhd_solver_cell_labels",
+    "qmhd_solver_cell_labels", "qmhd_solver_autotune", "qmhd_solver_set_tuning", "qmhd_solver_get_tuning",
+    "qmhd_batch_create", "qmhd_batch_destroy", "qmhd_batch_run", "qmhd_batch_run_status",
 ]
 
 MLP_FEATURES = 100
@@ -142,6 +143,15 @@ def load() -> C.CDLL:
     L.qmhd_solver_plain_faces.argtypes = [VP, C.POINTER(I64), C.POINTER(I64)]
     L.qmhd_solver_cell_features.argtypes = [VP, VP]
     L.qmhd_solver_cell_labels.argtypes = [VP, D, VP]
+    I32A = C.POINTER(C.c_int32)
+    L.qmhd_solver_autotune.argtypes = [VP, C.c_int, I32A]
+    L.qmhd_solver_set_tuning.argtypes = [VP, I32A]
+    L.qmhd_solver_get_tuning.argtypes = [VP, I32A]
+    L.qmhd_batch_create.argtypes = [C.POINTER(VP), C.c_int, C.POINTER(VP)]
+    L.qmhd_batch_destroy.argtypes = [VP]
+    L.qmhd_batch_destroy.restype = None
+    L.qmhd_batch_run.argtypes = [VP, C.c_int]
+    L.qmhd_batch_run_status.argtypes = [VP, C.c_int, C.POINTER(I64), C.POINTER(I64), C.POINTER(C.c_int), DP]
     _lib = L
     return L
 

@@ -1,0 +1,123 @@
+"""Batched runs: many independent MHD runs advanced by one HIP graph launch per
+step (include/quasar_mhd.h, qmhd_batch_*), and the seeded sweep of mixed-size
+Orszag-Tang and rotor runs used as the batched workload.
+
+Each run keeps its own solver (buffers sized for its grid).  The batch
+captures the auto-dt step of every run, each on the run's own stream forked
+from one master stream, into a graph; the accept/reject decision of each
+run's step is made on the device, so the graph replays without host
+synchronization.  Runs of the same size share one autotuned launch shape
+(qmhd_solver_autotune on the first run of that size).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Dict, List, Sequence
+
+import numpy as np
+
+from . import _lib, mhd
+
+SWEEP_SIZES = (256, 384, 512, 768, 1024)
+
+
+class Batch:
+    """Graph-captured stepping of independent solvers (they stay owned by the
+    caller and must be stepped only through the batch while it exists)."""
+
+    def __init__(self, solvers: Sequence["mhd.MhdSolver2D"]):
+        self._lib = _lib.load()
+        self._solvers = list(solvers)
+        arr = (C.c_void_p * len(self._solvers))(*[s._h.value for s in self._solvers])
+        h = C.c_void_p()
+        _lib.check(self._lib.qmhd_batch_create(arr, len(self._solvers), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.qmhd_batch_destroy(h)
+            self._h = None
+
+    def run(self, steps: int) -> None:
+        _lib.check(self._lib.qmhd_batch_run(self._h, int(steps)))
+
+    def status(self, run: int) -> Dict:
+        a, r = C.c_int64(), C.c_int64()
+        fatal, t = C.c_int(), C.c_double()
+        _lib.check(self._lib.qmhd_batch_run_status(self._h, int(run), C.byref(a), C.byref(r), C.byref(fatal),
+                                                   C.byref(t)))
+        return {"accepted": a.value, "rejected": r.value, "fatal": bool(fatal.value), "sim_time": t.value}
+
+
+def sweep_runs(count: int = 64, seed: int = 0, sizes=SWEEP_SIZES) -> List[Dict]:
+    """The seeded sweep: sizes drawn uniformly from `sizes` with a Philox
+    generator keyed by `seed`, Orszag-Tang and rotor alternating, each with a
+    seeded 1e-3 white-noise perturbation of rho, vx and vy (noise seed = run
+    index)."""
+    r = np.random.Generator(np.random.Philox(key=seed))
+    picks = r.integers(0, len(sizes), size=count)
+    return [{"run": k, "n": int(sizes[int(picks[k])]), "kind": "orszag_tang" if k % 2 == 0 else "rotor",
+             "noise": 1e-3, "noise_seed": k} for k in range(count)]
+
+
+def problem(kind: str, n: int, noise: float, noise_seed: int):
+    """(MhdInitialConditionSpec, boundary) of one sweep run."""
+    spec = mhd.MhdInitialConditionSpec()
+    spec.kind = kind
+    spec.grid = mhd.Grid2D(n, n, 1.0, 1.0, 0.0, 0.0, 4)
+    if kind == "orszag_tang":
+        spec.gamma = 5.0 / 3.0
+        spec.b_uniform = [1.0 / math.sqrt(4.0 * math.pi), 0.0, 0.0]
+        spec.ot_rho = 25.0 / (36.0 * math.pi)
+        spec.ot_p = 5.0 / (12.0 * math.pi)
+        sides = "periodic"
+    else:
+        spec.gamma = 1.4
+        spec.center = [0.5, 0.5]
+        spec.b_uniform = [5.0 / math.sqrt(4.0 * math.pi), 0.0, 0.0]
+        spec.p_ambient = 1.0
+        sides = "outflow"
+    spec.noise_amplitude = noise
+    spec.noise_seed = noise_seed
+    return spec, sides
+
+
+def make_run(desc: Dict, arithmetic: str = "fp32", indicator: str = "mlp", network=None):
+    spec, sides = problem(desc["kind"], desc["n"], desc["noise"], desc["noise_seed"])
+    cfg = mhd.MhdConfig()
+    cfg.grid = spec.grid
+    cfg.gamma = spec.gamma
+    cfg.reconstruction = "mp7"
+    cfg.arithmetic = arithmetic
+    cfg.indicator = indicator
+    cfg.boundary.set_fluid_all(sides)
+    cfg.boundary.set_field_all(sides)
+    s = mhd.MhdSolver2D(cfg)
+    for name, plane in mhd.build_initial_state(spec).items():
+        s.seed_state(name, plane)
+    if indicator == "mlp":
+        params, threshold = network
+        s.set_indicator(params, threshold)
+    return s
+
+
+def build_sweep(descs: Sequence[Dict], arithmetic: str = "fp32", indicator: str = "mlp", network=None,
+                autotune_steps: int = 0):
+    """Solvers of the sweep; with autotune_steps > 0, the first run of each
+    size is autotuned and the others of that size take its launch shape."""
+    if indicator == "mlp" and network is None:
+        from . import learned
+        p, t, _ = learned.load()
+        network = (p, t)
+    solvers, tuned = [], {}
+    for d in descs:
+        s = make_run(d, arithmetic, indicator, network)
+        if autotune_steps > 0:
+            if d["n"] not in tuned:
+                tuned[d["n"]] = s.autotune(autotune_steps)
+            else:
+                s.set_tuning(tuned[d["n"]])
+        solvers.append(s)
+    return solvers, tuned

@@ -1245,8 +1245,66 @@ __global__ void status_reset_kernel(StepStatus* s) {
   }
 }
 
+// Batched launcher: accept or reject the attempt on the device with the
+// host run loop's rule (python/quasar/mhd/cli.py:386-491 with
+// src/physics/mhd/mhd_solver.cpp:965-1143): the live state must be
+// admissible and discretely solenoidal and the CFL rate finite (otherwise the
+// run is stopped), and every stage of the attempt admissible (otherwise the
+// step is rejected and left to the host's positivity controller).
+__global__ void batch_commit_kernel(const StepStatus* __restrict__ st, double* __restrict__ acc) {
+  if (threadIdx.x != 0) return;
+  auto val = [](unsigned long long b) { return __longlong_as_double(static_cast<long long>(b)); };
+  const double eps = 2.220446049250313080847e-16;
+  const double res = val(st->divb_residual), scl = val(st->divb_scale);
+  double rel;
+  if (!(fin(res) && fin(scl)))
+    rel = Lim<double>::inf();
+  else if (res == 0.0)
+    rel = 0.0;
+  else
+    rel = scl > 0.0 ? res / scl : Lim<double>::inf();
+  const bool live_ok = fin(rel) && rel <= 1024.0 * eps && val(st->live_theta0) >= 1.0 && fin(val(st->max_rate));
+  const bool low_anchor = val(st->live_theta1) >= 1.0;
+  bool stages_ok = true;
+  for (int s = 0; s < 3; ++s) {
+    double th = val(st->theta[s]);
+    if (s == 2 && low_anchor) th = fmin(th, val(st->theta[3]));
+    stages_ok = stages_ok && fin(th) && th >= 1.0;
+  }
+  const bool fatal = acc[2] != 0.0 || !live_ok;
+  const bool accept = !fatal && stages_ok;
+  if (fatal) acc[2] = 1.0;
+  if (accept) {
+    acc[0] += 1.0;
+    acc[3] += st->dt;
+  } else if (!fatal) {
+    acc[1] += 1.0;
+  }
+  acc[4] = accept ? 1.0 : 0.0;
+}
+
+__global__ __launch_bounds__(256) void batch_restore_kernel(std::size_t n, const double* __restrict__ acc,
+                                                            StateView live, StateOut cand) {
+  if (acc[4] != 0.0) return;
+  for (std::size_t t = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) cand.c[c][t] = live.c[c][t];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers.
+
+void launch_batch_commit(const StepStatus* status, double* acc, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_commit_kernel, dim3(1), dim3(64), 0, stream, status, acc);
+}
+
+void launch_batch_restore(const GridD& g, const double* acc, const StateView& live, const StateOut& cand,
+                          hipStream_t stream) {
+  const std::size_t n = static_cast<std::size_t>(g.pitch) * (g.ny + 2 * g.ng);
+  hipLaunchKernelGGL(batch_restore_kernel, dim3(256), dim3(256), 0, stream, n, acc, live, cand);
+}
 
 void launch_eq_flags(const GridD& g, int dir, const IfaceView<double>& in, int* eq, hipStream_t stream) {
   const dim3 b(64, 4);
@@ -1334,7 +1392,8 @@ template <class T>
 void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma, const Flags4& flags,
                                 const StateView& u, const double* bxc, const double* byc, const uint8_t* cell_flags,
                                 const IfaceView<T>& out, int* eq, unsigned long long* fb_count,
-                                unsigned long long* fb_list, unsigned long long* plain_total, hipStream_t stream) {
+                                unsigned long long* fb_list, unsigned long long* plain_total, hipStream_t stream,
+                                int list_blocks) {
   const int lx = dir == 0 ? g.nx + 1 : g.nx + 2 * g.ng;
   const int ly = dir == 0 ? g.ny + 2 * g.ng : g.ny + 1;
   const int slo = dir == 0 ? flags.side[0] : flags.side[2];
@@ -1342,7 +1401,7 @@ void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma
   const T gamma_t = T(gamma);
   // The plain list can hold a large share of the faces: enough blocks to fill
   // the device at the MP kernel's occupancy.
-  const dim3 list_grid(1024), list_block(256);
+  const dim3 list_grid(static_cast<unsigned>(list_blocks)), list_block(256);
   hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb_count);
   const dim3 cb(kCheapTx, kCheapTy);
   hipLaunchKernelGGL((cheap_reconstruct_kernel<T>), blocks2d(lx, ly, cb), cb, 0, stream, g, dir, gamma_t, u, bxc, byc,
@@ -1359,9 +1418,8 @@ void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma
 template <class T>
 void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4& flags, double gamma,
                  const IfaceView<T>& in, const FluxView<T>& flux, double* rate, hipStream_t stream,
-                 const uint8_t* cell_flags) {
+                 const uint8_t* cell_flags, dim3 b) {
   const T gm = T(gamma);
-  const dim3 b(64, 4);
   const dim3 grid = blocks2d(g.nx + 1, g.ny + 1, b);
   const int periodic = dir == 0 ? (flags.side[0] == 0 && flags.side[1] == 0) : (flags.side[2] == 0 && flags.side[3] == 0);
   switch (order) {
@@ -1429,8 +1487,7 @@ void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamm
 
 template <class T>
 void launch_corner(const GridD& g, int order, const Flags4& flags, const FluxView<T>& fx, const FluxView<T>& fy,
-                   const double* cell_ez, const int* eqx, const int* eqy, double* ez, hipStream_t stream) {
-  const dim3 b(64, 4);
+                   const double* cell_ez, const int* eqx, const int* eqy, double* ez, hipStream_t stream, dim3 b) {
   hipLaunchKernelGGL((corner_kernel<T>), blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, order, flags, fx, fy,
                      cell_ez, eqx, eqy, ez);
 }
@@ -1438,8 +1495,7 @@ void launch_corner(const GridD& g, int order, const Flags4& flags, const FluxVie
 template <class T>
 void launch_update(const GridD& g, double a, double b_, double cw, const double* dt_dev, const StateView& un,
                    const StateView& us, const FluxView<T>& fx, const FluxView<T>& fy, const double* ez,
-                   const StateOut& out, hipStream_t stream) {
-  const dim3 b(64, 4);
+                   const StateOut& out, hipStream_t stream, dim3 b) {
   hipLaunchKernelGGL((update_kernel<T>), blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, a, b_, cw, dt_dev, un,
                      us, fx, fy, ez, out);
 }
@@ -1502,19 +1558,19 @@ hipError_t upload_quad_rows(const QuadRows& rows) {
                                       const double*, const double*, const IfaceView<T>&, int*, unsigned long long*, \
                                       unsigned long long*, hipStream_t);                                          \
   template void launch_flux<T>(const GridD&, int, int, int, const Flags4&, double, const IfaceView<T>&,            \
-                               const FluxView<T>&, double*, hipStream_t, const uint8_t*);                         \
+                               const FluxView<T>&, double*, hipStream_t, const uint8_t*, dim3);                   \
   template void launch_cell_emf<T>(const GridD&, int, const Flags4&, double, const StateView&, const double*,     \
                                    const double*, double*, hipStream_t, const uint8_t*);                          \
   template void launch_corner<T>(const GridD&, int, const Flags4&, const FluxView<T>&, const FluxView<T>&,        \
-                                 const double*, const int*, const int*, double*, hipStream_t);                    \
+                                 const double*, const int*, const int*, double*, hipStream_t, dim3);              \
   template void launch_update<T>(const GridD&, double, double, double, const double*, const StateView&,           \
                                  const StateView&, const FluxView<T>&, const FluxView<T>&, const double*,         \
-                                 const StateOut&, hipStream_t);
+                                 const StateOut&, hipStream_t, dim3);
 #define QMHD_INSTANTIATE_LEARNED(T)                                                                                \
   template void launch_reconstruct_learned<T>(const GridD&, int, int, double, const Flags4&, const StateView&,     \
                                               const double*, const double*, const uint8_t*, const IfaceView<T>&,  \
                                               int*, unsigned long long*, unsigned long long*, unsigned long long*, \
-                                              hipStream_t);
+                                              hipStream_t, int);
 QMHD_INSTANTIATE_LEARNED(double)
 QMHD_INSTANTIATE_LEARNED(float)
 #undef QMHD_INSTANTIATE_LEARNED

@@ -84,11 +84,12 @@ template <class T>
 void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma, const Flags4& flags,
                                 const StateView& u, const double* bxc, const double* byc, const uint8_t* cell_flags,
                                 const IfaceView<T>& out, int* eq, unsigned long long* fb_count,
-                                unsigned long long* fb_list, unsigned long long* plain_total, hipStream_t stream);
+                                unsigned long long* fb_list, unsigned long long* plain_total, hipStream_t stream,
+                                int list_blocks = 1024);
 template <class T>
 void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4& flags, double gamma,
                  const IfaceView<T>& in, const FluxView<T>& flux, double* rate, hipStream_t stream,
-                 const uint8_t* cell_flags = nullptr);
+                 const uint8_t* cell_flags = nullptr, dim3 block = dim3(64, 4));
 void launch_cfl_reduce(const GridD& g, const double* rate_x, const double* rate_y, unsigned long long* max_rate,
                        hipStream_t stream);
 template <class T>
@@ -97,11 +98,12 @@ void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamm
                      const uint8_t* cell_flags = nullptr);
 template <class T>
 void launch_corner(const GridD& g, int order, const Flags4& flags, const FluxView<T>& fx, const FluxView<T>& fy,
-                   const double* cell_ez, const int* eqx, const int* eqy, double* ez, hipStream_t stream);
+                   const double* cell_ez, const int* eqx, const int* eqy, double* ez, hipStream_t stream,
+                   dim3 block = dim3(64, 4));
 template <class T>
 void launch_update(const GridD& g, double a, double b, double cw, const double* dt_dev, const StateView& un,
                    const StateView& us, const FluxView<T>& fx, const FluxView<T>& fy, const double* ez,
-                   const StateOut& out, hipStream_t stream);
+                   const StateOut& out, hipStream_t stream, dim3 block = dim3(64, 4));
 void launch_rk_combine(const GridD& g, double a, double b, double c, const StateView& un, const StateView& us,
                        const StateView& dudt, const StateOut& out, hipStream_t stream);
 void launch_ghosts(const GridD& g, const Flags4& fluid, const Flags4& field, const StateOut& u, hipStream_t stream);
@@ -118,6 +120,14 @@ void launch_flux_difference(const GridD& g, int dir, const FluxView<double>& f, 
                             hipStream_t stream);
 void launch_curl(const GridD& g, const double* ez, double* dbx, double* dby, hipStream_t stream);
 void launch_status_reset(StepStatus* s, hipStream_t stream);
+// Device-side step commit of the batched launcher: acc[0] accepted steps,
+// acc[1] rejected steps, acc[2] fatal (live state invalid), acc[3] simulated
+// time, acc[4] the decision of the last step (1 accepted).  On rejection the
+// restore launch copies the live state over the candidate, so the next step
+// (which reads the candidate register) starts again from the same state.
+void launch_batch_commit(const StepStatus* status, double* acc, hipStream_t stream);
+void launch_batch_restore(const GridD& g, const double* acc, const StateView& live, const StateOut& cand,
+                          hipStream_t stream);
 
 // Learned form: troubled-cell indicator network (parameters: FP32 vector of
 // mlp::kParams; packed: BF16 MFMA operands from launch_mlp_pack), training

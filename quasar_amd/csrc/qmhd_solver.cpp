@@ -109,6 +109,7 @@ class KernelClock {
   }
   double total_ms(int k) const { return ms_[k]; }
   long long launches(int k) const { return count_[k]; }
+  bool enabled() const { return on_; }
 
  private:
   struct Pair {
@@ -509,7 +510,8 @@ class Solver {
   void issue_reconstruct(Register& r, int dir, int order, double* const* ip, int* eq) {
     if (learned_ && order == order_)
       launch_reconstruct_learned<T>(g_, dir, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, flags(),
-                                    iface_as<T>(ip), eq, fb_count(), fb_list(), plain_total(), stream_);
+                                    iface_as<T>(ip), eq, fb_count(), fb_list(), plain_total(), stream_,
+                                    tuning_.list_blocks);
     else
       launch_reconstruct<T>(g_, dir, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface_as<T>(ip), eq,
                             fb_count(), fb_list(), stream_);
@@ -521,12 +523,14 @@ class Solver {
     clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_,
                  [&] { issue_reconstruct<T>(r, 0, order, ip, reinterpret_cast<int*>(eqx_.p)); });
     clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
-      launch_flux<T>(g_, 0, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fxp), rate_x_.p, stream_, cf);
+      launch_flux<T>(g_, 0, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fxp), rate_x_.p, stream_, cf,
+                     tuning_.flux);
     });
     clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_,
                  [&] { issue_reconstruct<T>(r, 1, order, ip, reinterpret_cast<int*>(eqy_.p)); });
     clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
-      launch_flux<T>(g_, 1, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fyp), rate_y_.p, stream_, cf);
+      launch_flux<T>(g_, 1, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fyp), rate_y_.p, stream_, cf,
+                     tuning_.flux);
     });
   }
 
@@ -540,11 +544,12 @@ class Solver {
     });
     clock_.timed(QMHD_KERNEL_CORNER, stream_, [&] {
       launch_corner<T>(g_, order, field_, flux_as<T>(fxp), flux_as<T>(fyp), cell_ez_.p,
-                       reinterpret_cast<int*>(eqx_.p), reinterpret_cast<int*>(eqy_.p), ez_.p, stream_);
+                       reinterpret_cast<int*>(eqx_.p), reinterpret_cast<int*>(eqy_.p), ez_.p, stream_,
+                       tuning_.corner);
     });
     clock_.timed(QMHD_KERNEL_UPDATE, stream_, [&] {
       launch_update<T>(g_, a, b, c_host, dt_dev, reg_[L_].view(), us.view(), flux_as<T>(fxp), flux_as<T>(fyp),
-                       ez_.p, out.out(), stream_);
+                       ez_.p, out.out(), stream_, tuning_.update);
     });
   }
 
@@ -663,6 +668,128 @@ class Solver {
     QMHD_HIP(hipGetLastError());
     QMHD_HIP(hipStreamSynchronize(stream_));
   }
+  // -- launch tuning ---------------------------------------------------------
+  // Block shapes of the flux, corner and update launches and the grid of the
+  // persistent list launches (waves per CU of the plain path over its list).
+  struct Tuning {
+    dim3 flux{64, 4}, corner{64, 4}, update{64, 4};
+    int list_blocks{1024};
+  };
+  static bool valid_block(int bx, int by) {
+    return bx >= 1 && by >= 1 && bx * by <= 256 && (bx * by) % 64 == 0;
+  }
+  void set_tuning(const int32_t* t) {
+    if (!valid_block(t[0], t[1]) || !valid_block(t[2], t[3]) || !valid_block(t[4], t[5]) || t[6] < 1 ||
+        t[6] > 65535)
+      throw std::invalid_argument("MhdSolver2D: tuning needs blocks of 64..256 threads (multiples of 64) and "
+                                  "1..65535 list blocks");
+    tuning_.flux = dim3(t[0], t[1]);
+    tuning_.corner = dim3(t[2], t[3]);
+    tuning_.update = dim3(t[4], t[5]);
+    tuning_.list_blocks = t[6];
+  }
+  void get_tuning(int32_t* t) const {
+    t[0] = tuning_.flux.x, t[1] = tuning_.flux.y, t[2] = tuning_.corner.x, t[3] = tuning_.corner.y;
+    t[4] = tuning_.update.x, t[5] = tuning_.update.y, t[6] = tuning_.list_blocks, t[7] = 0;
+  }
+
+  // Times every candidate of each knob over `trial_steps` steps (after one
+  // warm-up step) with the kernel clock, keeps the fastest, and restores the
+  // live state: the run continues from where it was.
+  void autotune(int trial_steps) {
+    if (trial_steps < 1) throw std::invalid_argument("MhdSolver2D::autotune: trial_steps must be positive");
+    copy_live_to_backup();
+    QMHD_HIP(hipStreamSynchronize(stream_));
+    const int prev_owned = solver_owned_;
+    const dim3 shapes[] = {dim3(64, 4), dim3(32, 8), dim3(128, 2), dim3(256, 1), dim3(16, 16), dim3(64, 1)};
+    auto trial = [&](int kind) {
+      int done = 0;
+      double t = 0.0;
+      run(1, 0.0, &done, &t);
+      clock_.enable(true);
+      run(trial_steps, 0.0, &done, &t);
+      QMHD_HIP(hipStreamSynchronize(stream_));
+      clock_.harvest();
+      const double ms = clock_.total_ms(kind);
+      clock_.enable(false);
+      restore_backup();
+      solver_owned_ = prev_owned;
+      return ms;
+    };
+    dim3* knobs[3] = {&tuning_.flux, &tuning_.corner, &tuning_.update};
+    const int kinds[3] = {QMHD_KERNEL_FLUX, QMHD_KERNEL_CORNER, QMHD_KERNEL_UPDATE};
+    for (int k = 0; k < 3; ++k) {
+      dim3 best = *knobs[k];
+      double best_ms = std::numeric_limits<double>::infinity();
+      for (const dim3& c : shapes) {
+        *knobs[k] = c;
+        const double ms = trial(kinds[k]);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
+      }
+      *knobs[k] = best;
+    }
+    if (learned_) {
+      const int grids[] = {256, 512, 1024, 2048, 4096};
+      int best = tuning_.list_blocks;
+      double best_ms = std::numeric_limits<double>::infinity();
+      for (int c : grids) {
+        tuning_.list_blocks = c;
+        const double ms = trial(QMHD_KERNEL_RECONSTRUCT);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
+      }
+      tuning_.list_blocks = best;
+    }
+  }
+
+  // -- batched, graph-captured stepping ------------------------------------
+  // One auto-dt step with no host synchronization: the accept/reject decision
+  // and the commit happen on the device (acc = per-run accumulators), and the
+  // live/candidate roles swap on the host as bookkeeping only, so a capture of
+  // two consecutive steps replays forever.
+  double* batch_acc() {
+    if (!batch_acc_.p) {
+      batch_acc_ = DBuf(8);
+      QMHD_HIP(hipMemset(batch_acc_.p, 0, 8 * sizeof(double)));
+    }
+    return batch_acc_.p;
+  }
+  void issue_batch_step() {
+    if (learned_ && !indicator_set_)
+      throw std::invalid_argument("MhdSolver2D: indicator 'mlp' needs its network (set_indicator) before stepping");
+    double* acc = batch_acc();
+    const double eff = order_ >= 5 ? std::min(cfl_, 0.2) : cfl_;
+    issue_rate_phase(order_);
+    launch_dt(&status_->max_rate, eff, g_.dx, g_.dy, &status_->dt, nullptr, stream_);
+    issue_attempt(0.0, &status_->dt, 0, false);
+    launch_batch_commit(status_, acc, stream_);
+    launch_batch_restore(g_, acc, reg_[L_].view(), reg_[C_].out(), stream_);
+    QMHD_HIP(hipGetLastError());
+    swap_roles();
+  }
+  void swap_roles() {
+    std::swap(L_, C_);
+    cache_order_ = -1;
+    solver_owned_ = true;
+  }
+  void batch_status(long long* accepted, long long* rejected, int* fatal, double* sim_time) {
+    double h[8];
+    QMHD_HIP(hipMemcpyAsync(h, batch_acc(), sizeof h, hipMemcpyDeviceToHost, stream_));
+    QMHD_HIP(hipStreamSynchronize(stream_));
+    *accepted = static_cast<long long>(h[0]);
+    *rejected = static_cast<long long>(h[1]);
+    *fatal = h[2] != 0.0 ? 1 : 0;
+    *sim_time = h[3];
+  }
+  hipStream_t hip_stream() const { return stream_; }
+  bool profiling() const { return clock_.enabled(); }
+  int live_index() const { return L_; }
+
   void cell_labels(double tol, uint8_t* dev_out) {
     issue_live_prep();
     uint8_t* fx = reinterpret_cast<uint8_t*>(cell_ez_.p);
@@ -823,6 +950,8 @@ class Solver {
   float threshold_{0.f};
   long long faces_issued_{0};
   DBuf mlp_params_, mlp_packed_, cell_flags_, plain_total_;
+  Tuning tuning_;
+  DBuf batch_acc_;
   float* params() { return reinterpret_cast<float*>(mlp_params_.p); }
   uint16_t* packed() { return reinterpret_cast<uint16_t*>(mlp_packed_.p); }
   uint8_t* flags() { return reinterpret_cast<uint8_t*>(cell_flags_.p); }
@@ -1403,6 +1532,136 @@ int qmhd_cell_indicator(const qmhd_grid* g, double gamma, const double* const* u
 
 int qmhd_stream_synchronize(void* stream) {
   return guarded([&] { qmhd::hip_check(hipStreamSynchronize(st(stream)), "hipStreamSynchronize"); });
+}
+
+int qmhd_solver_autotune(qmhd_solver* s, int trial_steps, int32_t* tuning_out) {
+  return guarded([&] {
+    s->impl.autotune(trial_steps);
+    if (tuning_out) s->impl.get_tuning(tuning_out);
+  });
+}
+
+int qmhd_solver_set_tuning(qmhd_solver* s, const int32_t* tuning) {
+  return guarded([&] { s->impl.set_tuning(tuning); });
+}
+
+int qmhd_solver_get_tuning(const qmhd_solver* s, int32_t* tuning) {
+  return guarded([&] { s->impl.get_tuning(tuning); });
+}
+
+// ---- batched runs ---------------------------------------------------------
+
+struct qmhd_batch {
+  std::vector<qmhd::Solver*> runs;
+  hipStream_t master{nullptr};
+  hipEvent_t fork{nullptr};
+  std::vector<hipEvent_t> joins;
+  hipGraph_t graph[2]{nullptr, nullptr};
+  hipGraphExec_t exec[2]{nullptr, nullptr};
+  std::vector<int> live0;  // each run's live register index at parity 0
+  int parity{0};           // parity of the next step
+  int solver_parity{0};    // parity the runs' register roles reflect
+
+  ~qmhd_batch() {
+    for (int p = 0; p < 2; ++p) {
+      if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
+      if (graph[p]) (void)hipGraphDestroy(graph[p]);
+    }
+    for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (master) (void)hipStreamDestroy(master);
+  }
+
+  void align_roles(int want) {
+    if (solver_parity == want) return;
+    for (qmhd::Solver* r : runs) r->swap_roles();
+    solver_parity = want;
+  }
+
+  // Step of every run, each on its own stream, forked from and joined back
+  // to the master stream, captured as one graph per register parity.
+  void build() {
+    qmhd::hip_check(hipStreamCreate(&master), "hipStreamCreate");
+    qmhd::hip_check(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate");
+    joins.resize(runs.size());
+    for (auto& e : joins) qmhd::hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    for (qmhd::Solver* r : runs) {
+      if (r->profiling()) throw std::invalid_argument("qmhd_batch: turn kernel profiling off on every run");
+      (void)r->batch_acc();
+      qmhd::hip_check(hipStreamSynchronize(r->hip_stream()), "hipStreamSynchronize");
+      live0.push_back(r->live_index());
+    }
+    for (int p = 0; p < 2; ++p) {
+      qmhd::hip_check(hipStreamBeginCapture(master, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+      try {
+        qmhd::hip_check(hipEventRecord(fork, master), "hipEventRecord");
+        for (std::size_t k = 0; k < runs.size(); ++k) {
+          qmhd::hip_check(hipStreamWaitEvent(runs[k]->hip_stream(), fork, 0), "hipStreamWaitEvent");
+          runs[k]->issue_batch_step();
+          qmhd::hip_check(hipEventRecord(joins[k], runs[k]->hip_stream()), "hipEventRecord");
+          qmhd::hip_check(hipStreamWaitEvent(master, joins[k], 0), "hipStreamWaitEvent");
+        }
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(master, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      qmhd::hip_check(hipStreamEndCapture(master, &graph[p]), "hipStreamEndCapture");
+      qmhd::hip_check(hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    // Each capture swapped every run's roles once: back at parity 0.
+    solver_parity = 0;
+  }
+
+  void run(int steps) {
+    if (steps < 0) throw std::invalid_argument("qmhd_batch_run: steps must be non-negative");
+    align_roles(parity);
+    for (std::size_t k = 0; k < runs.size(); ++k) {
+      const int expect = parity == 0 ? live0[k] : -1;
+      if (expect >= 0 && runs[k]->live_index() != expect)
+        throw std::invalid_argument("qmhd_batch_run: a run was stepped outside the batch");
+    }
+    for (int k = 0; k < steps; ++k) {
+      qmhd::hip_check(hipGraphLaunch(exec[parity], master), "hipGraphLaunch");
+      parity ^= 1;
+    }
+    qmhd::hip_check(hipStreamSynchronize(master), "hipStreamSynchronize");
+    align_roles(parity);
+  }
+};
+
+int qmhd_batch_create(qmhd_solver* const* runs, int count, qmhd_batch** out) {
+  return guarded([&] {
+    if (count < 1 || !runs || !out) throw std::invalid_argument("qmhd_batch_create: need at least one run");
+    std::unique_ptr<qmhd_batch> b(new qmhd_batch);
+    for (int k = 0; k < count; ++k) {
+      if (!runs[k]) throw std::invalid_argument("qmhd_batch_create: null run");
+      for (int q = 0; q < k; ++q)
+        if (runs[q] == runs[k]) throw std::invalid_argument("qmhd_batch_create: a run appears twice");
+      b->runs.push_back(&runs[k]->impl);
+    }
+    b->build();
+    *out = b.release();
+  });
+}
+
+void qmhd_batch_destroy(qmhd_batch* b) { delete b; }
+
+int qmhd_batch_run(qmhd_batch* b, int steps) {
+  return guarded([&] { b->run(steps); });
+}
+
+int qmhd_batch_run_status(qmhd_batch* b, int run, int64_t* accepted, int64_t* rejected, int* fatal,
+                          double* sim_time) {
+  return guarded([&] {
+    if (run < 0 || run >= static_cast<int>(b->runs.size()))
+      throw std::invalid_argument("qmhd_batch_run_status: run index out of range");
+    long long a = 0, r = 0;
+    b->runs[run]->batch_status(&a, &r, fatal, sim_time);
+    *accepted = a;
+    *rejected = r;
+  });
 }
 
 int qmhd_solver_set_indicator(qmhd_solver* s, const float* host_params, float threshold_logit) {

@@ -368,6 +368,26 @@ class MhdSolver2D:
         """Label bits of the interior cells (nx*ny bytes) into device memory."""
         _lib.check(self._lib.qmhd_solver_cell_labels(self._h, float(label_tol), C.c_void_p(device_ptr)))
 
+    # -- launch tuning -------------------------------------------------------
+    TUNING_KEYS = ("flux_block_x", "flux_block_y", "corner_block_x", "corner_block_y", "update_block_x",
+                   "update_block_y", "list_blocks")
+
+    def autotune(self, trial_steps: int = 3) -> Dict[str, int]:
+        """Time every launch-shape candidate over trial_steps steps, keep the
+        fastest (the live state is restored) and return the choice."""
+        t = (C.c_int32 * 8)()
+        _lib.check(self._lib.qmhd_solver_autotune(self._h, int(trial_steps), t))
+        return dict(zip(self.TUNING_KEYS, list(t)))
+
+    def tuning(self) -> Dict[str, int]:
+        t = (C.c_int32 * 8)()
+        _lib.check(self._lib.qmhd_solver_get_tuning(self._h, t))
+        return dict(zip(self.TUNING_KEYS, list(t)))
+
+    def set_tuning(self, tuning: Dict[str, int]) -> None:
+        t = (C.c_int32 * 8)(*[int(tuning[k]) for k in self.TUNING_KEYS], 0)
+        _lib.check(self._lib.qmhd_solver_set_tuning(self._h, t))
+
     def set_profiling(self, enable: bool) -> None:
         """Bracket every launch with HIP events on the solver's stream (resets
         the accumulated times)."""
