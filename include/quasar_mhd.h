@@ -277,7 +277,8 @@ int qmhd_solver_set_indicator(qmhd_solver* s, const float* host_params, float th
 int qmhd_solver_plain_faces(qmhd_solver* s, int64_t* plain, int64_t* total);
 /* ---- launch tuning and batched runs (no reference counterpart) --------
  * Tuning vector (8 x int32): flux block x, y; corner block x, y; update
- * block x, y (64..256 threads, multiples of 64); blocks of the persistent
+ * block x, y (64..256 threads, multiples of 64; the MP5/MP7 corner launch is a
+ * fixed 16x16 tile and ignores its pair); blocks of the persistent
  * list launches of the learned form's plain path; reserved.
  * qmhd_solver_autotune times every candidate over `trial_steps` steps (the
  * live state is restored afterwards) and keeps the fastest. */

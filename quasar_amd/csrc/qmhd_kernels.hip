@@ -765,6 +765,88 @@ __global__ __launch_bounds__(256) void corner_kernel(GridD g, int order, Flags4 
   ez[idx(g, i, j)] = corner_ez<T>(g, order, m, fx, fy, cell_ez, eqx, eqy, i, j);
 }
 
+// MP5/MP7 corners, 16x16 per block.  A block whose corners all lie away from
+// the physical boundary (every index maps to itself) stages the cell E_z it
+// reads in LDS and forms the cell -> corner interpolation separably: the
+// x-direction edge interpolation of every tile row once, then the y-direction
+// one per corner, the same operations on the same values as corner_ez.  Other
+// blocks take corner_ez per corner.
+template <class T, int ORDER>
+__global__ __launch_bounds__(256) void corner_tiled_kernel(GridD g, Flags4 m, FluxView<T> fx, FluxView<T> fy,
+                                                           const double* __restrict__ cell_ez,
+                                                           const int* __restrict__ eqx, const int* __restrict__ eqy,
+                                                           double* __restrict__ ez) {
+  constexpr int LO = ORDER >= 7 ? -4 : -3;
+  constexpr int NT = ORDER >= 7 ? 8 : 6;
+  constexpr int RH = ORDER >= 7 ? 3 : 2;
+  constexpr int B = 16;
+  constexpr int S = B + NT - 1;  // tile span of cells
+  __shared__ double ce[S][S + 1];
+  __shared__ double line[S][B];
+  const int i0 = static_cast<int>(blockIdx.x) * B, j0 = static_cast<int>(blockIdx.y) * B;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * B + tx;
+  const int i = i0 + tx, j = j0 + ty;
+  // Interior test of corner_ez for the block's first and last corners.
+  const bool interior = i0 + LO - RH >= 0 && i0 + B - 1 + LO + NT - 1 + RH < g.nx && j0 + LO - RH >= 0 &&
+                        j0 + B - 1 + LO + NT - 1 + RH < g.ny && i0 > 0 && j0 > 0;
+  if (!interior) {
+    if (i <= g.nx && j <= g.ny) ez[idx(g, i, j)] = corner_ez<T>(g, ORDER, m, fx, fy, cell_ez, eqx, eqy, i, j);
+    return;
+  }
+  for (int t = tid; t < S * S; t += B * B) {
+    const int a = t % S, b = t / S;
+    ce[b][a] = cell_ez[idx(g, i0 + LO + a, j0 + LO + b)];
+  }
+  __syncthreads();
+  // line[r][c]: x-direction interpolation of tile row r onto corner column c.
+  for (int t = tid; t < S * B; t += B * B) {
+    const int r = t / B, c = t % B;
+    double e[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) e[k] = ce[r][c + k];
+    line[r][c] = edge_interp(ORDER, e);
+  }
+  __syncthreads();
+  double e[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) e[t] = -static_cast<double>(fx.c[6][idx(g, i, j + LO + t)]);
+  const double ex = edge_interp(ORDER, e);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) e[t] = static_cast<double>(fy.c[5][idx(g, i + LO + t, j)]);
+  const double ey = edge_interp(ORDER, e);
+#pragma unroll
+  for (int q = 0; q < NT; ++q) e[q] = line[ty + q][tx];
+  const double centered = edge_interp(ORDER, e);
+  // One-dimensional shortcuts: every face problem of the stencil line has no
+  // jump (faces j+LO-RH .. j+LO+NT-1+RH along the line) and the cell E_z is
+  // constant along the other axis.
+  auto no_jump = [&](const int* eq, int d) {
+    for (int o = LO - RH; o <= LO + NT - 1 + RH; ++o)
+      if (eq[d == 0 ? idx(g, i, j + o) : idx(g, i + o, j)] == 0) return false;
+    return true;
+  };
+  auto constant_along = [&](int axis) {
+    for (int v = 0; v < NT; ++v) {
+      const double ref = axis == 0 ? ce[ty + v][tx] : ce[ty][tx + v];
+      for (int w = 1; w < NT; ++w) {
+        const double c = axis == 0 ? ce[ty + v][tx + w] : ce[ty + w][tx + v];
+        if (c != ref) return false;
+      }
+    }
+    return true;
+  };
+  const bool x_only = no_jump(eqy, 1) && constant_along(1);
+  const bool y_only = no_jump(eqx, 0) && constant_along(0);
+  double r;
+  if (x_only && !y_only)
+    r = ex;
+  else if (y_only && !x_only)
+    r = ey;
+  else
+    r = (ex + ey) - centered;
+  ez[idx(g, i, j)] = r;
+}
+
 // ---------------------------------------------------------------------------
 // Fused residual + Shu-Osher stage: out = a Un + b Us + c L(U) with
 // L = -div F for the cell components and the EMF curl for the faces.  A zero
@@ -1488,6 +1570,16 @@ void launch_cell_emf(const GridD& g, int order, const Flags4& flags, double gamm
 template <class T>
 void launch_corner(const GridD& g, int order, const Flags4& flags, const FluxView<T>& fx, const FluxView<T>& fy,
                    const double* cell_ez, const int* eqx, const int* eqy, double* ez, hipStream_t stream, dim3 b) {
+  if (order == 5 || order == 7) {
+    const dim3 tb(16, 16);
+    if (order == 7)
+      hipLaunchKernelGGL((corner_tiled_kernel<T, 7>), blocks2d(g.nx + 1, g.ny + 1, tb), tb, 0, stream, g, flags, fx,
+                         fy, cell_ez, eqx, eqy, ez);
+    else
+      hipLaunchKernelGGL((corner_tiled_kernel<T, 5>), blocks2d(g.nx + 1, g.ny + 1, tb), tb, 0, stream, g, flags, fx,
+                         fy, cell_ez, eqx, eqy, ez);
+    return;
+  }
   hipLaunchKernelGGL((corner_kernel<T>), blocks2d(g.nx + 1, g.ny + 1, b), b, 0, stream, g, order, flags, fx, fy,
                      cell_ez, eqx, eqy, ez);
 }

@@ -716,9 +716,11 @@ class Solver {
       solver_owned_ = prev_owned;
       return ms;
     };
-    dim3* knobs[3] = {&tuning_.flux, &tuning_.corner, &tuning_.update};
-    const int kinds[3] = {QMHD_KERNEL_FLUX, QMHD_KERNEL_CORNER, QMHD_KERNEL_UPDATE};
-    for (int k = 0; k < 3; ++k) {
+    // MP5/MP7 corners run as fixed 16x16 LDS tiles: their shape is not a knob.
+    dim3* knobs[3] = {&tuning_.flux, &tuning_.update, &tuning_.corner};
+    const int kinds[3] = {QMHD_KERNEL_FLUX, QMHD_KERNEL_UPDATE, QMHD_KERNEL_CORNER};
+    const int nknobs = order_ >= 5 ? 2 : 3;
+    for (int k = 0; k < nknobs; ++k) {
       dim3 best = *knobs[k];
       double best_ms = std::numeric_limits<double>::infinity();
       for (const dim3& c : shapes) {
