@@ -127,13 +127,28 @@ __device__ __forceinline__ double ulp_of(double v) {
 // atomics.
 __device__ __forceinline__ double ok_or_zero(double v) { return fin(v) ? v : 0.0; }
 
-// Collocation of the staggered field (every padded cell), the admissible
-// fraction of a stage candidate against its base (status.theta[slot]; with
-// theta_both the other collocation into theta[3]) and the live-state
-// preflights (admissibility under both collocations, the relative div B
-// residual).  Each thread covers kRowsPerThread rows.
-__global__ __launch_bounds__(256) void prep_kernel(GridD g, StateView u, double* __restrict__ bxc,
-                                                   double* __restrict__ byc, StateView base,
+// Centred collocation of the staggered field on every padded cell whose
+// stencil lies inside the grid (the outer ghost layers take the one-sided
+// closures of ghost_collocate_kernel).
+__global__ __launch_bounds__(256) void colloc_centred_kernel(GridD g, const double* __restrict__ bxf,
+                                                             const double* __restrict__ byf,
+                                                             double* __restrict__ bxc, double* __restrict__ byc) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) - g.ng;
+  const int j = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y) - g.ng;
+  if (i >= g.nx + g.ng || j >= g.ny + g.ng) return;
+  const std::size_t k = idx(g, i, j);
+  double v;
+  if (face_to_cell_centred(g, bxf, 0, i, j, v)) bxc[k] = v;
+  if (face_to_cell_centred(g, byf, 1, j, i, v)) byc[k] = v;
+}
+
+// The admissible fraction of a stage candidate against its base
+// (status.theta[slot]; with theta_both the other collocation into theta[3])
+// and the live-state preflights (admissibility under both collocations, the
+// relative div B residual) over the interior cells, from the collocated
+// field.  Each thread covers kRowsPerThread rows.
+__global__ __launch_bounds__(256) void prep_kernel(GridD g, StateView u, const double* __restrict__ bxc,
+                                                   const double* __restrict__ byc, StateView base,
                                                    const double* __restrict__ base_bxc,
                                                    const double* __restrict__ base_byc, int theta_slot,
                                                    int theta_co, int theta_both, int live_checks,
@@ -143,18 +158,11 @@ __global__ __launch_bounds__(256) void prep_kernel(GridD g, StateView u, double*
   double th = 1.0, th_other = 1.0, live0 = 1.0, live1 = 1.0, res = 0.0, scl = 0.0;
   for (int r = 0; r < kRowsPerThread; ++r) {
     const int j = static_cast<int>((blockIdx.y * kRowsPerThread + r) * blockDim.y + threadIdx.y) - g.ng;
-    const bool in_pad = i < g.nx + g.ng && j < g.ny + g.ng;
-    double bxv = 0.0, byv = 0.0;
-    if (in_pad) {
-      // The outer ghost layers (one-sided closures) are filled by
-      // ghost_collocate_kernel; interior cells always take the centred stencil.
-      const std::size_t k = idx(g, i, j);
-      if (face_to_cell_centred(g, u.c[5], 0, i, j, bxv)) bxc[k] = bxv;
-      if (face_to_cell_centred(g, u.c[6], 1, j, i, byv)) byc[k] = byv;
-    }
-    const bool interior = in_pad && i >= 0 && j >= 0 && i < g.nx && j < g.ny;
+    const bool interior = i >= 0 && j >= 0 && i < g.nx && j < g.ny;
     if (interior) {
       const std::size_t k = idx(g, i, j);
+      // Collocated by colloc_centred_kernel (interior cells take the centred stencil).
+      const double bxv = bxc[k], byv = byc[k];
       const St<double> cu = cell_state(g, u, k, bxv, byv);
       if (theta_slot >= 0) {
         double t0 = 1.0, t1 = 1.0;
@@ -1418,12 +1426,16 @@ void launch_prep(const GridD& g, const StateView& u, double* bxc, double* byc, c
                  const double* base_bxc, const double* base_byc, int theta_slot, int theta_co, int theta_both,
                  int live_checks, int solver_owned, StepStatus* status, hipStream_t stream) {
   const dim3 b(64, 4);
-  hipLaunchKernelGGL(prep_kernel, blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, dim3(64, 4 * kRowsPerThread)), b, 0,
-                     stream, g, u, bxc, byc, base,
-                     base_bxc, base_byc, theta_slot, theta_co, theta_both, live_checks, solver_owned, status);
+  hipLaunchKernelGGL(colloc_centred_kernel, blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, b), b, 0, stream, g, u.c[5],
+                     u.c[6], bxc, byc);
   const int nghost = 2 * g.ng * (g.ny + 2 * g.ng) + 2 * g.ng * (g.nx + 2 * g.ng);
   hipLaunchKernelGGL(ghost_collocate_kernel, dim3((nghost + 255) / 256), dim3(256), 0, stream, g, u.c[5], u.c[6], bxc,
                      byc);
+  if (theta_slot < 0 && !live_checks) return;
+  // prep_kernel's grid starts at padded cell -ng: interior threads only.
+  hipLaunchKernelGGL(prep_kernel, blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, dim3(64, 4 * kRowsPerThread)), b, 0,
+                     stream, g, u, bxc, byc, base, base_bxc, base_byc, theta_slot, theta_co, theta_both, live_checks,
+                     solver_owned, status);
 }
 
 void launch_collocate(const GridD& g, int axis, int co, const double* face, double* cell, hipStream_t stream) {
