@@ -493,8 +493,16 @@ __device__ __forceinline__ void speed_parts(const St<T>& s, int dir, T gamma, T&
   }
   const T bn = dir == 0 ? s.bx : s.by;
   const T bt = dir == 0 ? s.by : s.bx;
-  fast = fast_speed_x(s.rho, p, bn, bt, s.bz, gamma);
-  vel = qabs(dir == 0 ? s.mx : s.my) / s.rho;
+  // One reciprocal for the sound speed, the field term and |v_n|.
+  const T inv_rho = T(1) / s.rho;
+  const T a2 = gamma * p * inv_rho;
+  const T b2 = (bn * bn + bt * bt + s.bz * s.bz) * inv_rho;
+  const T c2 = bn * bn * inv_rho;
+  const T sum = a2 + b2;
+  T disc = sum * sum - T(4) * a2 * c2;
+  disc = disc < T(0) ? T(0) : disc;
+  fast = qsqrt(T(0.5) * (sum + qsqrt(disc)));
+  vel = qabs(dir == 0 ? s.mx : s.my) * inv_rho;
   if (!(fin(vel) && fin(fast))) {
     vel = Lim<T>::inf();
     fast = Lim<T>::inf();

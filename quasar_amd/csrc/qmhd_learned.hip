@@ -309,11 +309,10 @@ __global__ __launch_bounds__(256) void indicator_kernel(GridD g, double gamma, S
   __syncthreads();
   const int wv = tid >> 6, lane = tid & 63, grp = lane >> 4, col = lane & 15;
   const bf16x4* pk = reinterpret_cast<const bf16x4*>(packed);
+#pragma nounroll
   for (int r = 0; r < 4; ++r) {
     const int row = 4 * wv + r;
-    bf16x4 xb[mlp::kKs1];
-#pragma unroll
-    for (int ks = 0; ks < mlp::kKs1; ++ks) {
+    auto gen = [&](int ks) {
       float v[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -325,10 +324,10 @@ __global__ __launch_bounds__(256) void indicator_kernel(GridD g, double gamma, S
           v[e] = mlp::feature_value(q[m][row + 2 + dy][col + 2 + dx], q[m][row + 2][col + 2], sc[m][row][col]);
         }
       }
-      xb[ks] = mlp::pack4(v[0], v[1], v[2], v[3]);
-    }
+      return mlp::pack4(v[0], v[1], v[2], v[3]);
+    };
     f32x4 z1[2], z2[2];
-    const float z = mlp::forward16(pk, p, xb, z1, z2);
+    const float z = mlp::forward16_gen(pk, p, gen, z1, z2);
     const int i = i0 + col, j = j0 + row;
     if (grp == 0 && i < g.nx + g.ng && j < g.ny + g.ng)
       flags[idx(g, i, j)] = (ok[row][col] && z < threshold) ? 0 : 1;

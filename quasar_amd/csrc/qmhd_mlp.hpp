@@ -139,26 +139,33 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
   return v;
 }
 
-// Forward pass of the 16 samples one wave holds.  xb[ks] is the B operand of
-// layer 1 (features f = 16ks + 4(lane/16) + e of sample lane%16).  Returns
-// the logit of sample lane%16 in every lane; z1/z2 receive the pre-activation
-// tiles of both hidden layers in the MFMA output layout (hidden unit
-// 16t + 4(lane/16) + e of sample lane%16).
-__device__ __forceinline__ float forward16(const bf16x4* __restrict__ pk, const float* __restrict__ p,
-                                           const bf16x4 xb[kKs1], f32x4 z1[2], f32x4 z2[2]) {
+// Forward pass of the 16 samples one wave holds.  gen(ks) returns the B
+// operand of layer 1 for k-step ks (features f = 16ks + 4(lane/16) + e of
+// sample lane%16); both hidden-unit tiles consume it before the next one is
+// formed.  Returns the logit of sample lane%16 in every lane; z1/z2 receive
+// the pre-activation tiles of both hidden layers in the MFMA output layout
+// (hidden unit 16t + 4(lane/16) + e of sample lane%16).
+template <class G>
+__device__ __forceinline__ float forward16_gen(const bf16x4* __restrict__ pk, const float* __restrict__ p, G gen,
+                                               f32x4 z1[2], f32x4 z2[2]) {
   const int lane = static_cast<int>(__lane_id());
   const int grp = lane >> 4;
   const bf16x4* l1 = pk;
   const bf16x4* l2 = pk + kPackL1 / 4;
+  f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < kKs1; ++ks) acc = mfma16(l1[(t * kKs1 + ks) * 64 + lane], xb[ks], acc);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) acc[e] += p[kOffB1 + 16 * t + 4 * grp + e];
-    z1[t] = acc;
+  for (int ks = 0; ks < kKs1; ++ks) {
+    const bf16x4 x = gen(ks);
+    a0 = mfma16(l1[ks * 64 + lane], x, a0);
+    a1 = mfma16(l1[(kKs1 + ks) * 64 + lane], x, a1);
   }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    a0[e] += p[kOffB1 + 4 * grp + e];
+    a1[e] += p[kOffB1 + 16 + 4 * grp + e];
+  }
+  z1[0] = a0;
+  z1[1] = a1;
   bf16x4 hb[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
@@ -179,6 +186,12 @@ __device__ __forceinline__ float forward16(const bf16x4* __restrict__ pk, const 
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
   return part + p[kOffB3];
+}
+
+// As forward16_gen with the layer-1 operands given.
+__device__ __forceinline__ float forward16(const bf16x4* __restrict__ pk, const float* __restrict__ p,
+                                           const bf16x4 xb[kKs1], f32x4 z1[2], f32x4 z2[2]) {
+  return forward16_gen(pk, p, [&](int ks) { return xb[ks]; }, z1, z2);
 }
 
 }  // namespace mlp

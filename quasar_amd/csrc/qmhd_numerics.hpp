@@ -55,6 +55,7 @@ template <>
 struct Lim<double> {
   static constexpr double eps = 2.220446049250313080847e-16;
   static constexpr double tiny = 2.2250738585072014e-308;
+  static constexpr double half_max_pow2 = 0x1p1023;  // largest power of two
   static __host__ __device__ __forceinline__ double inf() { return __builtin_huge_val(); }
   static __host__ __device__ __forceinline__ double nan() { return __builtin_nan(""); }
 };
@@ -62,6 +63,7 @@ template <>
 struct Lim<float> {
   static constexpr float eps = 1.1920928955078125e-7f;
   static constexpr float tiny = 1.17549435e-38f;
+  static constexpr float half_max_pow2 = 0x1p127f;
   static __host__ __device__ __forceinline__ float inf() { return __builtin_huge_valf(); }
   static __host__ __device__ __forceinline__ float nan() { return __builtin_nanf(""); }
 };
@@ -279,6 +281,16 @@ __host__ __device__ __forceinline__ St<T> to_cons(const Prim<T>& w, T gamma) {
 // evaluation and a rounding margin: when the fast value is farther from zero
 // than 32 epsilon of the magnitudes it cancels, the reference form has the
 // same sign; otherwise the reference form decides.
+// fin(m / rho) for finite m and finite rho > 0, without the division in the
+// common case: |m| < rho * 2^emax (the product is exact or overflows to
+// infinity, and then the quotient is at most max/2) bounds |m / rho| below
+// 2^emax; otherwise the quotient itself is tested.
+template <class T>
+__host__ __device__ __forceinline__ bool finite_ratio(T m, T rho) {
+  if (qabs(m) < rho * Lim<T>::half_max_pow2) return true;
+  return fin(m / rho);
+}
+
 template <class T>
 __host__ __device__ __forceinline__ bool pressure_positive(const St<T>& u, T gamma) {
   const T kin = T(0.5) * (u.mx * u.mx + u.my * u.my + u.mz * u.mz) / u.rho;

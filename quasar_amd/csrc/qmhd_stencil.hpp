@@ -407,7 +407,7 @@ template <class T>
 __host__ __device__ __forceinline__ bool admissible_state(const St<T>& s, T gamma) {
   if (!(fin(s.rho) && s.rho > T(0))) return false;
   if (!(fin(s.mx) && fin(s.my) && fin(s.mz) && fin(s.en) && fin(s.bx) && fin(s.by) && fin(s.bz))) return false;
-  if (!(fin(s.mx / s.rho) && fin(s.my / s.rho) && fin(s.mz / s.rho))) return false;
+  if (!(finite_ratio(s.mx, s.rho) && finite_ratio(s.my, s.rho) && finite_ratio(s.mz, s.rho))) return false;
   return pressure_positive(s, gamma);
 }
 
