@@ -129,7 +129,9 @@ __device__ __forceinline__ double ok_or_zero(double v) { return fin(v) ? v : 0.0
 
 // Centred collocation of the staggered field on every padded cell whose
 // stencil lies inside the grid (the outer ghost layers take the one-sided
-// closures of ghost_collocate_kernel).
+// closures of ghost_collocate_kernel).  EXACT: correctly rounded (the FP64
+// form); otherwise a fused multiply-add sum (the FP32 forms).
+template <bool EXACT>
 __global__ __launch_bounds__(256) void colloc_centred_kernel(GridD g, const double* __restrict__ bxf,
                                                              const double* __restrict__ byf,
                                                              double* __restrict__ bxc, double* __restrict__ byc) {
@@ -138,8 +140,13 @@ __global__ __launch_bounds__(256) void colloc_centred_kernel(GridD g, const doub
   if (i >= g.nx + g.ng || j >= g.ny + g.ng) return;
   const std::size_t k = idx(g, i, j);
   double v;
-  if (face_to_cell_centred(g, bxf, 0, i, j, v)) bxc[k] = v;
-  if (face_to_cell_centred(g, byf, 1, j, i, v)) byc[k] = v;
+  if (EXACT) {
+    if (face_to_cell_centred(g, bxf, 0, i, j, v)) bxc[k] = v;
+    if (face_to_cell_centred(g, byf, 1, j, i, v)) byc[k] = v;
+  } else {
+    if (face_to_cell_centred_fma(g, bxf, 0, i, j, v)) bxc[k] = v;
+    if (face_to_cell_centred_fma(g, byf, 1, j, i, v)) byc[k] = v;
+  }
 }
 
 // The admissible fraction of a stage candidate against its base
@@ -525,13 +532,10 @@ __device__ __forceinline__ T face_speed(const St<T>& L, const St<T>& R, int dir,
 // one HLL flux at its face-average states (second order across the face, as
 // accurate as needed where the indicator finds the flow smooth) and its rate
 // from those states.
-template <class T, int ORDER, bool LEARNED = false>
-__global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_only, int periodic_dir, T gamma,
-                                                   IfaceView<T> in, FluxView<T> flux, double* __restrict__ rate,
-                                                   const uint8_t* __restrict__ cell_flags) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i > g.nx || j > g.ny) return;
+template <class T, int ORDER, bool LEARNED>
+__device__ __forceinline__ void flux_face(const GridD& g, int dir, int hll_only, int periodic_dir, T gamma,
+                                          const IfaceView<T>& in, const FluxView<T>& flux, double* __restrict__ rate,
+                                          const uint8_t* __restrict__ cell_flags, int i, int j) {
   int si = i, sj = j;
   if (periodic_dir) {
     if (dir == 0 && i == g.nx) si = 0;
@@ -629,6 +633,16 @@ __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_onl
   for (int c = 0; c < 8; ++c) flux.c[c][ko] = comp(f, c);
   const double sp = static_cast<double>(smax);
   rate[ko] = (fin(sp) && sp >= 0.0) ? (sp * 0.5) / spacing : Lim<double>::inf();
+}
+
+template <class T, int ORDER, bool LEARNED = false>
+__global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_only, int periodic_dir, T gamma,
+                                                   IfaceView<T> in, FluxView<T> flux, double* __restrict__ rate,
+                                                   const uint8_t* __restrict__ cell_flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  flux_face<T, ORDER, LEARNED>(g, dir, hll_only, periodic_dir, gamma, in, flux, rate, cell_flags, i, j);
 }
 
 // Additive cell CFL rate from the incident face rates, reduced to the maximum.
@@ -1432,10 +1446,14 @@ inline dim3 blocks2d(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x,
 
 void launch_prep(const GridD& g, const StateView& u, double* bxc, double* byc, const StateView& base,
                  const double* base_bxc, const double* base_byc, int theta_slot, int theta_co, int theta_both,
-                 int live_checks, int solver_owned, StepStatus* status, hipStream_t stream) {
+                 int live_checks, int solver_owned, StepStatus* status, hipStream_t stream, bool exact) {
   const dim3 b(64, 4);
-  hipLaunchKernelGGL(colloc_centred_kernel, blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, b), b, 0, stream, g, u.c[5],
-                     u.c[6], bxc, byc);
+  if (exact)
+    hipLaunchKernelGGL((colloc_centred_kernel<true>), blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, b), b, 0, stream, g,
+                       u.c[5], u.c[6], bxc, byc);
+  else
+    hipLaunchKernelGGL((colloc_centred_kernel<false>), blocks2d(g.nx + 2 * g.ng, g.ny + 2 * g.ng, b), b, 0, stream, g,
+                       u.c[5], u.c[6], bxc, byc);
   const int nghost = 2 * g.ng * (g.ny + 2 * g.ng) + 2 * g.ng * (g.nx + 2 * g.ng);
   hipLaunchKernelGGL(ghost_collocate_kernel, dim3((nghost + 255) / 256), dim3(256), 0, stream, g, u.c[5], u.c[6], bxc,
                      byc);

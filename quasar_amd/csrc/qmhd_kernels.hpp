@@ -68,7 +68,7 @@ void build_quad_rows(QuadRows& rows);
 // Launchers (all asynchronous on `stream`).
 void launch_prep(const GridD& g, const StateView& u, double* bxc, double* byc, const StateView& base,
                  const double* base_bxc, const double* base_byc, int theta_slot, int theta_co, int theta_both,
-                 int live_checks, int solver_owned, StepStatus* status, hipStream_t stream);
+                 int live_checks, int solver_owned, StepStatus* status, hipStream_t stream, bool exact = true);
 void launch_collocate(const GridD& g, int axis, int co, const double* face, double* cell, hipStream_t stream);
 // The face-level launchers are instantiated for T = double (the FP64 form)
 // and T = float (the FP32 form: interface states, fluxes and their arithmetic

@@ -600,7 +600,7 @@ class Solver {
       clock_.timed(QMHD_KERNEL_GHOSTS, stream_, [&] { launch_ghosts(g_, fluid_, field_, out.out(), stream_); });
       clock_.timed(QMHD_KERNEL_PREP, stream_, [&] {
         launch_prep(g_, out.view(), out.bxc.p, out.byc.p, reg_[L_].view(), reg_[L_].bxc.p, reg_[L_].byc.p, s,
-                    pos_order, s == 2 ? 1 : 0, 0, 0, status_, stream_);
+                    pos_order, s == 2 ? 1 : 0, 0, 0, status_, stream_, !fp32_);
       });
     }
     QMHD_HIP(hipGetLastError());
@@ -612,7 +612,7 @@ class Solver {
     clock_.timed(QMHD_KERNEL_GHOSTS, stream_, [&] { launch_ghosts(g_, fluid_, field_, reg_[L_].out(), stream_); });
     clock_.timed(QMHD_KERNEL_PREP, stream_, [&] {
       launch_prep(g_, reg_[L_].view(), reg_[L_].bxc.p, reg_[L_].byc.p, reg_[L_].view(), reg_[L_].bxc.p,
-                  reg_[L_].byc.p, -1, 0, 0, 1, solver_owned_ ? 1 : 0, status_, stream_);
+                  reg_[L_].byc.p, -1, 0, 0, 1, solver_owned_ ? 1 : 0, status_, stream_, !fp32_);
     });
   }
 

@@ -281,6 +281,41 @@ __host__ __device__ __forceinline__ bool face_to_cell_centred(const GridD& g, co
   return true;
 }
 
+// The centred rule of face_to_cell_centred with a plain fused multiply-add
+// sum (the FP32 forms' collocation: not correctly rounded).
+__host__ __device__ __forceinline__ bool face_to_cell_centred_fma(const GridD& g, const double* f, int axis, int cell,
+                                                                  int fixed, double& out) {
+  const int n = axis == 0 ? g.nx : g.ny;
+  const int width = colloc_width(n + 2 * g.ng, g.ng);
+  const int lo = -g.ng, hi = n + g.ng - 1;
+  if (width == 1) {
+    out = axis_face(g, f, axis, cell, fixed);
+    return true;
+  }
+  const int start_c = cell - (width / 2 - 1);
+  if (!(start_c >= lo && start_c + width - 1 <= hi)) return false;
+  double acc = 0.0;
+  if (width == 8) {
+    const double w[8] = {-191.0, 1879.0, -9531.0, 68323.0, 68323.0, -9531.0, 1879.0, -191.0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc = fma(w[k], axis_face(g, f, axis, start_c + k, fixed), acc);
+    out = acc * (1.0 / 120960.0);
+  } else if (width == 6) {
+    const double w[6] = {11.0, -93.0, 802.0, 802.0, -93.0, 11.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc = fma(w[k], axis_face(g, f, axis, start_c + k, fixed), acc);
+    out = acc * (1.0 / 1440.0);
+  } else if (width == 4) {
+    const double w[4] = {-1.0, 13.0, 13.0, -1.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc = fma(w[k], axis_face(g, f, axis, start_c + k, fixed), acc);
+    out = acc * (1.0 / 24.0);
+  } else {
+    out = 0.5 * (axis_face(g, f, axis, start_c, fixed) + axis_face(g, f, axis, start_c + 1, fixed));
+  }
+  return true;
+}
+
 // The one-sided closure of width W with the expansion kept in registers (the
 // same correctly rounded value as one_sided_average); for the ghost-layer
 // kernel, where register use does not limit anything else.
