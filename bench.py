@@ -78,6 +78,10 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=256, help="cells per side of the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP32 and FP32+learned runs")
+    ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
+                    help="autotune every solver's launch shapes over STEPS steps per candidate before timing")
+    ap.add_argument("--only-form", choices=("fp32", "fp32_learned"),
+                    help="time only this form (the line's value and kernel shares are that form's; for profiling)")
     ap.add_argument("--sweep", action="store_true",
                     help="instead of the single-run benchmark: the batched sweep of 64 mixed-size runs "
                          "(FP32 + learned, one graph launch per step, autotuned launch shapes)")
@@ -110,6 +114,9 @@ def orszag_tang(n: int, noise: float = 0.0):
     return spec
 
 
+AUTOTUNE_STEPS = 0
+
+
 def make_solver(n: int, arithmetic: str = "fp64", indicator: str = "none", noise: float = 0.0):
     from quasar_amd import mhd
     spec = orszag_tang(n, noise)
@@ -130,6 +137,9 @@ def make_solver(n: int, arithmetic: str = "fp64", indicator: str = "none", noise
         from quasar_amd import learned
         params, threshold, _ = learned.load()
         solver.set_indicator(params, threshold)
+    if AUTOTUNE_STEPS > 0:
+        tuning = solver.autotune(AUTOTUNE_STEPS)
+        print(f"bench: autotuned {arithmetic}/{indicator}: {tuning}", file=sys.stderr, flush=True)
     return solver
 
 
@@ -292,9 +302,15 @@ def measured_traffic(kind: str, n: int):
         return None
 
 
+# Kernels of each timed kind of the FP64 MP7 step (one dispatch of each per
+# launch of the kind).
 KERNEL_SYMBOLS = {
-    "prep": "prep_kernel", "reconstruct": "reconstruct_kernel<double, 7>", "flux": "flux_kernel<double, 7>",
-    "cell_emf": "cell_emf_kernel<double, 7>", "corner": "corner_kernel", "update": "update_kernel",
+    "prep": ("colloc_centred_kernel<true>", "ghost_collocate_kernel", "qmhd::prep_kernel"),
+    "reconstruct": ("reconstruct_kernel<double, 7>", "reconstruct_fallback_kernel<double, 7>"),
+    "flux": ("flux_kernel<double, 7, false>",),
+    "cell_emf": ("cell_emf_kernel<double, 7, false>",),
+    "corner": ("corner_tiled_kernel<double, 7>",),
+    "update": ("update_kernel<double>",),
 }
 
 
@@ -336,14 +352,15 @@ def summarize_counters(fetch_dir: str, write_dir: str, n: int, probe_mib: int) -
     fetch, write = _means(frows), _means(wrows)
     out = {"probe_mib": probe_mib, "probe_fetch_kib": fprobe, "probe_write_kib": wprobe,
            "fetch_scale": fetch_scale, "write_scale": write_scale, "unit": "bytes per launch"}
-    for kind, sym in KERNEL_SYMBOLS.items():
-        names = [k for k in fetch if sym in k]
-        if not names:
+    for kind, syms in KERNEL_SYMBOLS.items():
+        names = [next((k for k in fetch if sym in k), None) for sym in syms]
+        if any(nm is None for nm in names):
             continue
-        name = names[0]
-        out[kind] = round((fetch[name] * fetch_scale + write.get(name, 0.0) * write_scale) * 1024.0)
-        out[kind + "_fetch_bytes"] = round(fetch[name] * fetch_scale * 1024.0)
-        out[kind + "_write_bytes"] = round(write.get(name, 0.0) * write_scale * 1024.0)
+        fb = sum(fetch[nm] for nm in names) * fetch_scale * 1024.0
+        wb = sum(write.get(nm, 0.0) for nm in names) * write_scale * 1024.0
+        out[kind] = round(fb + wb)
+        out[kind + "_fetch_bytes"] = round(fb)
+        out[kind + "_write_bytes"] = round(wb)
     return out
 
 
@@ -376,7 +393,9 @@ def cpu_baseline(n: int, budget_s: float):
 
 
 def main():
+    global AUTOTUNE_STEPS
     args = parse()
+    AUTOTUNE_STEPS = args.autotune
     if args.summarize_counters:
         summary = summarize_counters(args.summarize_counters[0], args.summarize_counters[1], args.n,
                                      args.copy_probe_mib or 1024)
@@ -402,6 +421,19 @@ def main():
         print(json.dumps(sweep(args)), flush=True)
         return
     n = args.n
+    if args.only_form:
+        arith, ind = ("fp32", "none") if args.only_form == "fp32" else ("fp32", "mlp")
+        print(f"bench: seeding Orszag-Tang {n}^2 MP7, form {args.only_form}", file=sys.stderr, flush=True)
+        s = make_solver(n, arith, ind, args.noise)
+        if args.warmup > 0:
+            s.run(args.warmup)
+        s.set_profiling(True)
+        w, st = timed_run(s, args.steps)
+        times = s.kernel_times()
+        print(json.dumps({"form": args.only_form, "value": round(n * n * args.steps / w, 1), "unit": "zone-updates/s",
+                          "ms_per_step": round(w / args.steps * 1e3, 4), "steps": args.steps, "warmup": args.warmup,
+                          "sim_time": st, "kernel_ms": {k: round(v[0], 3) for k, v in times.items()}}), flush=True)
+        return
     print(f"bench: seeding Orszag-Tang {n}^2 MP7 FP64", file=sys.stderr, flush=True)
     solver = make_solver(n, noise=args.noise)
     if args.copy_probe_mib > 0:

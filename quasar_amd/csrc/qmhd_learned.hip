@@ -16,9 +16,7 @@ namespace qmhd {
 using mlp::bf16x4;
 using mlp::f32x4;
 
-namespace {
-
-inline dim3 grid2d(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+static inline dim3 grid2d(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
 
 // ---------------------------------------------------------------------------
 // Network kernels.
@@ -374,8 +372,6 @@ __global__ __launch_bounds__(256) void cell_label_kernel(GridD g, const uint8_t*
   out[static_cast<std::size_t>(j) * g.nx + i] =
       fx[idx(g, i, j)] | fx[idx(g, i + 1, j)] | fy[idx(g, i, j)] | fy[idx(g, i, j + 1)];
 }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // Launchers.
