@@ -734,21 +734,21 @@ __global__ __launch_bounds__(256) void cell_emf_kernel(GridD g, int phys_x_high,
       for (int c = 0; c < 8; ++c) xrec[c][r][col] = point_from_averages<T, TW>(&tile[c][r][col], wx);
     }
     __syncthreads();
-    // Pass 2: y-direction recovery of this cell's NQ nodes at x node qx.
+    // Pass 2: y-direction recovery of this cell's NQ nodes at x node qx, one
+    // component at a time (7 samples live instead of 56).
     if (active && ok) {
-      T s[8][TW];
+      St<T> st[NQ];
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
+      for (int c = 0; c < 8; ++c) {
+        T sc[TW];
 #pragma unroll
-        for (int r = 0; r < TW; ++r) s[c][r] = xrec[c][threadIdx.y + r][threadIdx.x];
+        for (int r = 0; r < TW; ++r) sc[r] = xrec[c][threadIdx.y + r][threadIdx.x];
 #pragma unroll
-      for (int qy = 0; qy < NQ; ++qy) {
-        const double* wy = ORDER == 5 ? c_rows.mp5_point[qy] : c_rows.mp7_point[qy];
-        St<T> st;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) set_comp(st, c, point_from_averages<T, TW>(s[c], wy));
-        ok = ok && point_ez(st, gamma, point[qy][qx]);
+        for (int qy = 0; qy < NQ; ++qy)
+          set_comp(st[qy], c, point_from_averages<T, TW>(sc, ORDER == 5 ? c_rows.mp5_point[qy] : c_rows.mp7_point[qy]));
       }
+#pragma unroll
+      for (int qy = 0; qy < NQ; ++qy) ok = ok && point_ez(st[qy], gamma, point[qy][qx]);
     }
     __syncthreads();
   }
