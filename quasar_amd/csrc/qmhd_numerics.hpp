@@ -520,8 +520,10 @@ __host__ __device__ __forceinline__ void basis_build(const St<T>& s, T gamma, Ch
 #pragma unroll
     for (int v = 0; v < 7; ++v) R[v][k] = col[v];
   }
+  // Row by row, in a rolled loop: one row of L live beside R (keeps the
+  // reconstruction kernel at two waves per SIMD).
   bool usable = true;
-#pragma unroll
+#pragma nounroll
   for (int i = 0; i < 7; ++i) {
     T r[7], l[7], row[7];
     basis_rows(B, i, r, l);
