@@ -15,6 +15,7 @@
 #include "qmhd_stencil.hpp"
 
 #include <cstdint>
+#include <stdexcept>
 
 namespace qmhd {
 
@@ -470,14 +471,14 @@ __global__ void plain_faces_accumulate_kernel(const unsigned long long* __restri
 // ---------------------------------------------------------------------------
 // Face flux with transverse Gauss quadrature and the CFL face rate.
 
-// hll_only: the order-1 Lax-Friedrichs flux; cheap: the learned form's HLL
-// flux (Davis speeds); otherwise HLLD.
-template <class T>
-__device__ __forceinline__ St<T> riemann(const St<T>& L, const St<T>& R, int dir, int hll_only, T gamma,
-                                         bool cheap = false) {
+// LF: the order-1 Lax-Friedrichs flux (a launch parameter, resolved at
+// compile time so an HLLD kernel carries no other solver); cheap: the learned
+// form's HLL flux (Davis speeds); otherwise HLLD.
+template <bool LF, class T>
+__device__ __forceinline__ St<T> riemann(const St<T>& L, const St<T>& R, int dir, T gamma, bool cheap = false) {
   const St<T> lr = rot(L, dir), rr = rot(R, dir);
   St<T> f;
-  if (hll_only)
+  if (LF)
     f = lf_flux(lr, rr, gamma);
   else if (cheap)
     f = hll_davis_flux(lr, rr, gamma);
@@ -532,8 +533,8 @@ __device__ __forceinline__ T face_speed(const St<T>& L, const St<T>& R, int dir,
 // one HLL flux at its face-average states (second order across the face, as
 // accurate as needed where the indicator finds the flow smooth) and its rate
 // from those states.
-template <class T, int ORDER, bool LEARNED>
-__device__ __forceinline__ void flux_face(const GridD& g, int dir, int hll_only, int periodic_dir, T gamma,
+template <class T, int ORDER, bool LEARNED, bool LF>
+__device__ __forceinline__ void flux_face(const GridD& g, int dir, int periodic_dir, T gamma,
                                           const IfaceView<T>& in, const FluxView<T>& flux, double* __restrict__ rate,
                                           const uint8_t* __restrict__ cell_flags, int i, int j) {
   int si = i, sj = j;
@@ -555,10 +556,10 @@ __device__ __forceinline__ void flux_face(const GridD& g, int dir, int hll_only,
   St<T> f;
   T smax = face_speed(baseL, baseR, dir, ORDER, gamma);
   if (ORDER != 5 && ORDER != 7) {
-    f = riemann(baseL, baseR, dir, hll_only, gamma, cheap);
+    f = riemann<LF>(baseL, baseR, dir, gamma, cheap);
   } else if (LEARNED && cheap) {
     // Learned form, smooth face: one HLL flux at the face-average states.
-    f = riemann(baseL, baseR, dir, hll_only, gamma, true);
+    f = riemann<LF>(baseL, baseR, dir, gamma, true);
   } else {
     constexpr int H = ORDER == 5 ? 2 : 3;
     constexpr int NQ = ORDER == 5 ? 3 : 4;
@@ -600,34 +601,38 @@ __device__ __forceinline__ void flux_face(const GridD& g, int dir, int hll_only,
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
       if (q < first_bad) smax = fmax(smax, face_speed(pl[q], pr[q], dir, ORDER, gamma));
+    // Gauss average of the point fluxes, accumulated point by point in the
+    // same summation order, with the exact constant-flux shortcut (all point
+    // fluxes equal -> that flux) tracked as a running flag.  With an
+    // inadmissible point the face takes the flux of its face averages: one
+    // pass of the same loop at weight 1 (w * v = v exactly), so the kernel
+    // holds one solver per node and no separate one for that case.
     if (!ok) {
-      f = riemann(baseL, baseR, dir, hll_only, gamma, cheap);
-    } else {
-      // Gauss average of the point fluxes, accumulated point by point in the
-      // same summation order, with the exact constant-flux shortcut (all
-      // point fluxes equal -> that flux) tracked as a running flag.
-      T acc[8], first[8];
-      bool same[8];
+      pl[0] = baseL;
+      pr[0] = baseR;
+    }
+    T acc[8], first[8];
+    bool same[8];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const St<T> fq = riemann(pl[q], pr[q], dir, hll_only, gamma, cheap);
-        const T w = T(ORDER == 5 ? c_rows.mp5_gauss[q] : c_rows.mp7_gauss[q]);
+    for (int q = 0; q < NQ; ++q) {
+      if (q > 0 && !ok) break;
+      const St<T> fq = riemann<LF>(pl[q], pr[q], dir, gamma);
+      const T w = ok ? T(ORDER == 5 ? c_rows.mp5_gauss[q] : c_rows.mp7_gauss[q]) : T(1);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const T v = comp(fq, c);
-          if (q == 0) {
-            first[c] = v;
-            same[c] = v == v;
-            acc[c] = w * v;
-          } else {
-            same[c] = same[c] && v == first[c];
-            acc[c] += w * v;
-          }
+      for (int c = 0; c < 8; ++c) {
+        const T v = comp(fq, c);
+        if (q == 0) {
+          first[c] = v;
+          same[c] = v == v;
+          acc[c] = w * v;
+        } else {
+          same[c] = same[c] && v == first[c];
+          acc[c] += w * v;
         }
       }
-#pragma unroll
-      for (int c = 0; c < 8; ++c) set_comp(f, c, same[c] ? first[c] : acc[c]);
     }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) set_comp(f, c, same[c] ? first[c] : acc[c]);
   }
 #pragma unroll
   for (int c = 0; c < 8; ++c) flux.c[c][ko] = comp(f, c);
@@ -635,14 +640,14 @@ __device__ __forceinline__ void flux_face(const GridD& g, int dir, int hll_only,
   rate[ko] = (fin(sp) && sp >= 0.0) ? (sp * 0.5) / spacing : Lim<double>::inf();
 }
 
-template <class T, int ORDER, bool LEARNED = false>
-__global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int hll_only, int periodic_dir, T gamma,
+template <class T, int ORDER, bool LEARNED, bool LF>
+__global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int periodic_dir, T gamma,
                                                    IfaceView<T> in, FluxView<T> flux, double* __restrict__ rate,
                                                    const uint8_t* __restrict__ cell_flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.nx || j > g.ny) return;
-  flux_face<T, ORDER, LEARNED>(g, dir, hll_only, periodic_dir, gamma, in, flux, rate, cell_flags, i, j);
+  flux_face<T, ORDER, LEARNED, LF>(g, dir, periodic_dir, gamma, in, flux, rate, cell_flags, i, j);
 }
 
 // Additive cell CFL rate from the incident face rates, reduced to the maximum.
@@ -1542,28 +1547,29 @@ void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4&
   const T gm = T(gamma);
   const dim3 grid = blocks2d(g.nx + 1, g.ny + 1, b);
   const int periodic = dir == 0 ? (flags.side[0] == 0 && flags.side[1] == 0) : (flags.side[2] == 0 && flags.side[3] == 0);
+  if (cell_flags && (hll_only || (order != 5 && order != 7)))
+    throw std::invalid_argument("flux launch: the learned form runs the MP5/MP7 HLLD flux");
+  auto one_lane = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, b, 0, stream, g, dir, periodic, gm, in, flux, rate, cell_flags);
+  };
   switch (order) {
     case 1:
-      hipLaunchKernelGGL((flux_kernel<T, 1>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate, cell_flags);
+      hll_only ? one_lane(flux_kernel<T, 1, false, true>) : one_lane(flux_kernel<T, 1, false, false>);
       break;
     case 2:
-      hipLaunchKernelGGL((flux_kernel<T, 2>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate, cell_flags);
+      hll_only ? one_lane(flux_kernel<T, 2, false, true>) : one_lane(flux_kernel<T, 2, false, false>);
       break;
     case 5:
       if (cell_flags)
-        hipLaunchKernelGGL((flux_kernel<T, 5, true>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate,
-                           cell_flags);
+        one_lane(flux_kernel<T, 5, true, false>);
       else
-        hipLaunchKernelGGL((flux_kernel<T, 5>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate,
-                           cell_flags);
+        hll_only ? one_lane(flux_kernel<T, 5, false, true>) : one_lane(flux_kernel<T, 5, false, false>);
       break;
     default:
       if (cell_flags)
-        hipLaunchKernelGGL((flux_kernel<T, 7, true>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate,
-                           cell_flags);
+        one_lane(flux_kernel<T, 7, true, false>);
       else
-        hipLaunchKernelGGL((flux_kernel<T, 7>), grid, b, 0, stream, g, dir, hll_only, periodic, gm, in, flux, rate,
-                           cell_flags);
+        hll_only ? one_lane(flux_kernel<T, 7, false, true>) : one_lane(flux_kernel<T, 7, false, false>);
       break;
   }
 }

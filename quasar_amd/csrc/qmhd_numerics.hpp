@@ -616,12 +616,11 @@ __host__ __device__ __forceinline__ void basis_from_char(const CharBasis<T>& B, 
 }
 
 // ---------------------------------------------------------------------------
-// Fluxes along +x.
+// Fluxes along +x.  phys_flux_of takes 1/rho and the pressure of `u` when the
+// caller already holds them (the same values phys_flux computes).
 template <class T>
-__host__ __device__ __forceinline__ St<T> phys_flux(const St<T>& u, T gm1) {
-  const T inv = T(1) / u.rho;
+__host__ __device__ __forceinline__ St<T> phys_flux_of(const St<T>& u, T inv, T p) {
   const T vx = u.mx * inv, vy = u.my * inv, vz = u.mz * inv;
-  const T p = pressure(u, gm1);
   const T hbn2 = T(0.5) * u.bx * u.bx;
   const T hbt2 = T(0.5) * (u.by * u.by + u.bz * u.bz);
   St<T> f;
@@ -638,9 +637,14 @@ __host__ __device__ __forceinline__ St<T> phys_flux(const St<T>& u, T gm1) {
 }
 
 template <class T>
-__host__ __device__ __forceinline__ St<T> hll_flux(const St<T>& L, const St<T>& R, T sl, T sr, T gm1) {
-  const St<T> fl = phys_flux(L, gm1);
-  const St<T> fr = phys_flux(R, gm1);
+__host__ __device__ __forceinline__ St<T> phys_flux(const St<T>& u, T gm1) {
+  return phys_flux_of(u, T(1) / u.rho, pressure(u, gm1));
+}
+
+// HLL flux from the two physical fluxes.
+template <class T>
+__host__ __device__ __forceinline__ St<T> hll_combine(const St<T>& L, const St<T>& R, const St<T>& fl,
+                                                      const St<T>& fr, T sl, T sr) {
   if (sl >= T(0)) return fl;
   if (sr <= T(0)) return fr;
   const T scale = fmax(qabs(sl), qabs(sr));
@@ -659,6 +663,11 @@ __host__ __device__ __forceinline__ St<T> hll_flux(const St<T>& L, const St<T>& 
   f.by = wl * fl.by + wr * fr.by + diss * (R.by - L.by);
   f.bz = wl * fl.bz + wr * fr.bz + diss * (R.bz - L.bz);
   return f;
+}
+
+template <class T>
+__host__ __device__ __forceinline__ St<T> hll_flux(const St<T>& L, const St<T>& R, T sl, T sr, T gm1) {
+  return hll_combine(L, R, phys_flux(L, gm1), phys_flux(R, gm1), sl, sr);
 }
 
 template <class T>
@@ -700,12 +709,18 @@ __host__ __device__ __forceinline__ bool resolved(T value, T scale) {
   return fin(value) && qabs(value) > T(64) * Lim<T>::eps * (scale + Lim<T>::tiny);
 }
 
+// HLLD (Miyoshi & Kusano 2005) in the reference's operation order
+// (include/quasar/numerics/hlld_core.hpp:213).  Every exit to the HLL flux goes
+// through one call site and every physical flux through one evaluation, so the
+// four solves a face makes stay small enough for the instruction cache; each
+// outcome is computed by the same operations as a solver written with early
+// returns.
 template <class T>
 __host__ __device__ __forceinline__ St<T> hlld_flux(const St<T>& Lin, const St<T>& Rin, T gamma) {
   const T gm1 = gamma - T(1);
-  if (Lin.rho == Rin.rho && Lin.mx == Rin.mx && Lin.my == Rin.my && Lin.mz == Rin.mz &&
-      Lin.en == Rin.en && Lin.bx == Rin.bx && Lin.by == Rin.by && Lin.bz == Rin.bz)
-    return phys_flux(Lin, gm1);
+  const bool equal = Lin.rho == Rin.rho && Lin.mx == Rin.mx && Lin.my == Rin.my && Lin.mz == Rin.mz &&
+                     Lin.en == Rin.en && Lin.bx == Rin.bx && Lin.by == Rin.by && Lin.bz == Rin.bz;
+  if (equal) return phys_flux(Lin, gm1);
   const T bn = T(0.5) * (Lin.bx + Rin.bx);
   St<T> L = Lin, R = Rin;
   const T hbn2 = T(0.5) * bn * bn;
@@ -725,119 +740,123 @@ __host__ __device__ __forceinline__ St<T> hlld_flux(const St<T>& Lin, const St<T
   const T cfmax = cfL > cfR ? cfL : cfR;
   const T sl = (vxL < vxR ? vxL : vxR) - cfmax;
   const T sr = (vxL > vxR ? vxL : vxR) + cfmax;
-  if (sl >= T(0)) return phys_flux(L, gm1);
-  if (sr <= T(0)) return phys_flux(R, gm1);
-  if (!fin(bn2) || !fin(ptL) || !fin(ptR)) return hll_flux(L, R, sl, sr, gm1);
-  const T aR = (sr - vxR) * R.rho;
-  const T aL = (sl - vxL) * L.rho;
-  const T denomM = aR - aL;
-  if (!resolved(denomM, qabs(aR) + qabs(aL))) return hll_flux(L, R, sl, sr, gm1);
-  const T inv_dM = T(1) / denomM;
-  const T sm = (aR * vxR - aL * vxL - ptR + ptL) * inv_dM;
-  if (!fin(sm)) return hll_flux(L, R, sl, sr, gm1);
-  const T pts = (aR * ptL - aL * ptR + aL * aR * (vxR - vxL)) * inv_dM;
-  if (!fin(pts)) return hll_flux(L, R, sl, sr, gm1);
-  const T dLs = sl - sm, dRs = sr - sm;
-  const T wave_scale = qabs(sl) + qabs(sr) + qabs(sm);
-  if (!resolved(dLs, wave_scale) || !resolved(dRs, wave_scale)) return hll_flux(L, R, sl, sr, gm1);
-  const T rhoLs = aL / dLs;
-  const T rhoRs = aR / dRs;
-  if (!(rhoLs > T(0)) || !(rhoRs > T(0))) return hll_flux(L, R, sl, sr, gm1);
 
-  // Single-star states.
-  T vyLs, vzLs, byLs, bzLs, eLs, vyRs, vzRs, byRs, bzRs, eRs;
-  {
-    const T smv = sl - vxL;
-    const T q = L.rho * smv * dLs;
-    const T den = q - bn2;
-    if (!resolved(den, qabs(q) + bn2)) return hll_flux(L, R, sl, sr, gm1);
-    const T id = T(1) / den;
-    const T fac = bn * (sm - vxL) * id;
-    vyLs = vyL - L.by * fac;
-    vzLs = vzL - L.bz * fac;
-    const T bc = (L.rho * smv * smv - bn2) * id;
-    byLs = L.by * bc;
-    bzLs = L.bz * bc;
-    const T vdb = vxL * L.bx + vyL * L.by + vzL * L.bz;
-    const T vdbs = sm * bn + vyLs * byLs + vzLs * bzLs;
-    eLs = (smv * L.en - ptL * vxL + pts * sm + bn * (vdb - vdbs)) / dLs;
-  }
-  {
-    const T smv = sr - vxR;
-    const T q = R.rho * smv * dRs;
-    const T den = q - bn2;
-    if (!resolved(den, qabs(q) + bn2)) return hll_flux(L, R, sl, sr, gm1);
-    const T id = T(1) / den;
-    const T fac = bn * (sm - vxR) * id;
-    vyRs = vyR - R.by * fac;
-    vzRs = vzR - R.bz * fac;
-    const T bc = (R.rho * smv * smv - bn2) * id;
-    byRs = R.by * bc;
-    bzRs = R.bz * bc;
-    const T vdb = vxR * R.bx + vyR * R.by + vzR * R.bz;
-    const T vdbs = sm * bn + vyRs * byRs + vzRs * bzRs;
-    eRs = (smv * R.en - ptR * vxR + pts * sm + bn * (vdb - vdbs)) / dRs;
-  }
-  const St<T> usL{rhoLs, rhoLs * sm, rhoLs * vyLs, rhoLs * vzLs, eLs, bn, byLs, bzLs};
-  const St<T> usR{rhoRs, rhoRs * sm, rhoRs * vyRs, rhoRs * vzRs, eRs, bn, byRs, bzRs};
-  const T sqL = qsqrt(rhoLs), sqR = qsqrt(rhoRs);
-  const T abn = qabs(bn);
-  const T saL = sm - abn / sqL;
-  const T saR = sm + abn / sqR;
-  const bool bn_negligible =
-      bn2 <= T(64) * Lim<T>::eps * (qabs(pts) + qabs(ptL) + qabs(ptR) + Lim<T>::tiny);
+  // The fan: true with `f` set, false for the HLL flux.  Supersonic faces take
+  // the upwind physical flux (f = F(U), region coefficients zero).
   St<T> f;
-  if (bn_negligible) {
-    const bool left = sl <= T(0) && T(0) <= sm;
-    const St<T> fU = left ? phys_flux(L, gm1) : phys_flux(R, gm1);
-    const St<T>& U = left ? L : R;
-    const St<T>& us = left ? usL : usR;
-    const T s = left ? sl : sr;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) set_comp(f, c, comp(fU, c) + s * (comp(us, c) - comp(U, c)));
-    f.bx = T(0);
-    if (!fin(f.rho) || !fin(f.en)) return hll_flux(L, R, sl, sr, gm1);
-    return f;
-  }
-  if (T(0) <= saL || !(T(0) <= saR)) {
-    // Outer regions: F*_{L,R} = F_{L,R} + S_{L,R} (U* - U).
-    const bool left = T(0) <= saL;
-    const St<T> fU = left ? phys_flux(L, gm1) : phys_flux(R, gm1);
-    const St<T>& U = left ? L : R;
-    const St<T>& us = left ? usL : usR;
-    const T s = left ? sl : sr;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) set_comp(f, c, comp(fU, c) + s * (comp(us, c) - comp(U, c)));
-    f.bx = T(0);
-  } else {
-    const T inv_sum = T(1) / (sqL + sqR);
-    const T sg = bn >= T(0) ? T(1) : T(-1);
-    const T vyss = (sqL * vyLs + sqR * vyRs + (byRs - byLs) * sg) * inv_sum;
-    const T vzss = (sqL * vzLs + sqR * vzRs + (bzRs - bzLs) * sg) * inv_sum;
-    const T byss = (sqL * byRs + sqR * byLs + sqL * sqR * (vyRs - vyLs) * sg) * inv_sum;
-    const T bzss = (sqL * bzRs + sqR * bzLs + sqL * sqR * (vzRs - vzLs) * sg) * inv_sum;
-    const T vdbss = sm * bn + vyss * byss + vzss * bzss;
-    const bool left = T(0) <= sm;
-    const St<T>& U = left ? L : R;
-    const St<T>& us = left ? usL : usR;
-    const T s = left ? sl : sr;
-    const T sa = left ? saL : saR;
-    const T sq = left ? sqL : sqR;
-    const T vys = left ? vyLs : vyRs, vzs = left ? vzLs : vzRs;
-    const T bys = left ? byLs : byRs, bzs = left ? bzLs : bzRs;
-    const T vdbs = sm * bn + vys * bys + vzs * bzs;
-    const T side = left ? T(-1) : T(1);
-    const St<T> uss{us.rho, us.rho * sm, us.rho * vyss, us.rho * vzss,
+  auto fan = [&]() -> bool {
+    const bool supersonic = sl >= T(0) || sr <= T(0);
+    bool left = sl >= T(0);
+    T s = T(0), sa = T(0);
+    St<T> us{}, uss{};
+    bool inner = false, negligible = false;
+    if (!supersonic) {
+      if (!fin(bn2) || !fin(ptL) || !fin(ptR)) return false;
+      const T aR = (sr - vxR) * R.rho;
+      const T aL = (sl - vxL) * L.rho;
+      const T denomM = aR - aL;
+      if (!resolved(denomM, qabs(aR) + qabs(aL))) return false;
+      const T inv_dM = T(1) / denomM;
+      const T sm = (aR * vxR - aL * vxL - ptR + ptL) * inv_dM;
+      if (!fin(sm)) return false;
+      const T pts = (aR * ptL - aL * ptR + aL * aR * (vxR - vxL)) * inv_dM;
+      if (!fin(pts)) return false;
+      const T dLs = sl - sm, dRs = sr - sm;
+      const T wave_scale = qabs(sl) + qabs(sr) + qabs(sm);
+      if (!resolved(dLs, wave_scale) || !resolved(dRs, wave_scale)) return false;
+      const T rhoLs = aL / dLs;
+      const T rhoRs = aR / dRs;
+      if (!(rhoLs > T(0)) || !(rhoRs > T(0))) return false;
+
+      // Single-star states.
+      T vyLs, vzLs, byLs, bzLs, eLs, vyRs, vzRs, byRs, bzRs, eRs;
+      {
+        const T smv = sl - vxL;
+        const T q = L.rho * smv * dLs;
+        const T den = q - bn2;
+        if (!resolved(den, qabs(q) + bn2)) return false;
+        const T id = T(1) / den;
+        const T fac = bn * (sm - vxL) * id;
+        vyLs = vyL - L.by * fac;
+        vzLs = vzL - L.bz * fac;
+        const T bc = (L.rho * smv * smv - bn2) * id;
+        byLs = L.by * bc;
+        bzLs = L.bz * bc;
+        const T vdb = vxL * L.bx + vyL * L.by + vzL * L.bz;
+        const T vdbs = sm * bn + vyLs * byLs + vzLs * bzLs;
+        eLs = (smv * L.en - ptL * vxL + pts * sm + bn * (vdb - vdbs)) / dLs;
+      }
+      {
+        const T smv = sr - vxR;
+        const T q = R.rho * smv * dRs;
+        const T den = q - bn2;
+        if (!resolved(den, qabs(q) + bn2)) return false;
+        const T id = T(1) / den;
+        const T fac = bn * (sm - vxR) * id;
+        vyRs = vyR - R.by * fac;
+        vzRs = vzR - R.bz * fac;
+        const T bc = (R.rho * smv * smv - bn2) * id;
+        byRs = R.by * bc;
+        bzRs = R.bz * bc;
+        const T vdb = vxR * R.bx + vyR * R.by + vzR * R.bz;
+        const T vdbs = sm * bn + vyRs * byRs + vzRs * bzRs;
+        eRs = (smv * R.en - ptR * vxR + pts * sm + bn * (vdb - vdbs)) / dRs;
+      }
+      const T sqL = qsqrt(rhoLs), sqR = qsqrt(rhoRs);
+      const T abn = qabs(bn);
+      const T saL = sm - abn / sqL;
+      const T saR = sm + abn / sqR;
+      negligible = bn2 <= T(64) * Lim<T>::eps * (qabs(pts) + qabs(ptL) + qabs(ptR) + Lim<T>::tiny);
+      if (negligible)
+        left = sl <= T(0) && T(0) <= sm;
+      else if (T(0) <= saL || !(T(0) <= saR))
+        left = T(0) <= saL;  // outer regions: F*_{L,R} = F_{L,R} + S_{L,R} (U* - U)
+      else {
+        inner = true;
+        left = T(0) <= sm;
+      }
+      const St<T> usL{rhoLs, rhoLs * sm, rhoLs * vyLs, rhoLs * vzLs, eLs, bn, byLs, bzLs};
+      const St<T> usR{rhoRs, rhoRs * sm, rhoRs * vyRs, rhoRs * vzRs, eRs, bn, byRs, bzRs};
+      us = left ? usL : usR;
+      s = left ? sl : sr;
+      if (inner) {
+        const T inv_sum = T(1) / (sqL + sqR);
+        const T sg = bn >= T(0) ? T(1) : T(-1);
+        const T vyss = (sqL * vyLs + sqR * vyRs + (byRs - byLs) * sg) * inv_sum;
+        const T vzss = (sqL * vzLs + sqR * vzRs + (bzRs - bzLs) * sg) * inv_sum;
+        const T byss = (sqL * byRs + sqR * byLs + sqL * sqR * (vyRs - vyLs) * sg) * inv_sum;
+        const T bzss = (sqL * bzRs + sqR * bzLs + sqL * sqR * (vzRs - vzLs) * sg) * inv_sum;
+        const T vdbss = sm * bn + vyss * byss + vzss * bzss;
+        sa = left ? saL : saR;
+        const T sq = left ? sqL : sqR;
+        const T vys = left ? vyLs : vyRs, vzs = left ? vzLs : vzRs;
+        const T bys = left ? byLs : byRs, bzs = left ? bzLs : bzRs;
+        const T vdbs = sm * bn + vys * bys + vzs * bzs;
+        const T side = left ? T(-1) : T(1);
+        uss = St<T>{us.rho, us.rho * sm, us.rho * vyss, us.rho * vzss,
                     us.en + side * sq * sg * (vdbs - vdbss), bn, byss, bzss};
-    const St<T> fU = phys_flux(U, gm1);
+      }
+    }
+    const St<T>& U = left ? L : R;
+    const St<T> fU = phys_flux_of(U, left ? irL : irR, left ? pL : pR);
+    if (supersonic) {
+      f = fU;
+      return true;
+    }
+    if (inner) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
-      set_comp(f, c, comp(fU, c) + sa * (comp(uss, c) - comp(us, c)) + s * (comp(us, c) - comp(U, c)));
+      for (int c = 0; c < 8; ++c)
+        set_comp(f, c, comp(fU, c) + sa * (comp(uss, c) - comp(us, c)) + s * (comp(us, c) - comp(U, c)));
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) set_comp(f, c, comp(fU, c) + s * (comp(us, c) - comp(U, c)));
+    }
     f.bx = T(0);
-  }
-  if (!fin(f.rho) || !fin(f.mx) || !fin(f.my) || !fin(f.mz) || !fin(f.en) || !fin(f.by) || !fin(f.bz))
-    return hll_flux(L, R, sl, sr, gm1);
-  return f;
+    if (negligible) return fin(f.rho) && fin(f.en);
+    return fin(f.rho) && fin(f.mx) && fin(f.my) && fin(f.mz) && fin(f.en) && fin(f.by) && fin(f.bz);
+  };
+  if (fan()) return f;
+  return hll_combine(L, R, phys_flux_of(L, irL, pL), phys_flux_of(R, irR, pR), sl, sr);
 }
 
 // ---------------------------------------------------------------------------
