@@ -48,7 +48,8 @@ typedef struct {
 
 /* Mirrors quasar::mhd::MhdInitialConditionSpec
  * (include/quasar/physics/mhd/initial_conditions.hpp:62-124), kind 0 brio_wu,
- * 1 alfven_wave, 2 orszag_tang, 3 blast, 4 rotor.  Extensions: ot_rho/ot_p
+ * 1 alfven_wave, 2 orszag_tang, 3 blast, 4 rotor, 5 confined_blob (square of
+ * half-width blob_half about `center`).  Extensions: ot_rho/ot_p
  * (Orszag-Tang background density/pressure; 0 keeps gamma^2/gamma) and a
  * seeded Philox-4x32-10 white-noise perturbation of amplitude
  * noise_amplitude on rho (relative), vx and vy. */
@@ -65,6 +66,7 @@ typedef struct {
   double ot_rho, ot_p;
   double noise_amplitude;
   uint64_t noise_seed;
+  double blob_half;
 } qmhd_ic_spec;
 
 const char* qmhd_last_error(void);

@@ -29,6 +29,7 @@ typedef struct {
   double ot_rho, ot_p;
   double noise_amplitude;
   uint64_t noise_seed;
+  double blob_half;
 } OrcIcSpec;
 
 typedef struct {

@@ -1339,6 +1339,16 @@ int seed_initial_state(const Grid& g, const IcSpec& s, Field& out) {
           w.bz = s.b_uniform[2];
           break;
         }
+        case 5: {  // confined_blob :190-207; the in-plane field is exactly zero
+          const bool inside = std::fabs(xc - s.center[0]) <= s.blob_half &&
+                              std::fabs(yc - s.center[1]) <= s.blob_half;
+          w.rho = inside ? s.rho_in : s.rho_out;
+          w.p = inside ? s.p_in : s.p_out;
+          w.bx = Real{0};
+          w.by = Real{0};
+          w.bz = s.b_uniform[2];
+          break;
+        }
         default:
           return -1;
       }

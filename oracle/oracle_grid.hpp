@@ -111,7 +111,7 @@ void conserved_totals(const Grid& g, const Field& u, Real* mass, Real* energy);
 
 // Analytic initial conditions (src/backend/hip/mhd/mhd_initial_conditions.hip).
 struct IcSpec {
-  int kind{0};  // 0 brio_wu, 1 alfven_wave, 2 orszag_tang, 3 blast, 4 rotor
+  int kind{0};  // 0 brio_wu, 1 alfven_wave, 2 orszag_tang, 3 blast, 4 rotor, 5 confined_blob
   Real gamma{Real{5} / Real{3}};
   Real magnetic_scale{1};
   Real interface_x{0.5};
@@ -129,6 +129,7 @@ struct IcSpec {
   // Seeded white-noise perturbation on rho, vx, vy (0 = none).
   Real noise_amplitude{0};
   std::uint64_t noise_seed{0};
+  Real blob_half{0.25};  // confined_blob square half-width
 };
 int seed_initial_state(const Grid& g, const IcSpec& s, Field& out);
 

@@ -543,6 +543,7 @@ int orc_seed_initial_state(const OrcGrid* g, const OrcIcSpec* p, double* const* 
   s.ot_p = p->ot_p;
   s.noise_amplitude = p->noise_amplitude;
   s.noise_seed = p->noise_seed;
+  s.blob_half = p->blob_half;
   Field out = to_field(out8);
   return seed_initial_state(to_grid(g), s, out);
 }

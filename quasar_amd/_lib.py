@@ -31,7 +31,8 @@ class QIcSpec(C.Structure):
                 ("r_in", D), ("r0", D), ("r1", D), ("rho_in", D), ("rho_out", D),
                 ("p_in", D), ("p_out", D), ("p_core", D), ("p_ambient", D),
                 ("rho_ambient", D), ("u0", D),
-                ("ot_rho", D), ("ot_p", D), ("noise_amplitude", D), ("noise_seed", C.c_uint64)]
+                ("ot_rho", D), ("ot_p", D), ("noise_amplitude", D), ("noise_seed", C.c_uint64),
+                ("blob_half", D)]
 
 
 Name32 = C.c_char * 32

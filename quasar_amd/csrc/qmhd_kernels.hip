@@ -1142,6 +1142,13 @@ __global__ __launch_bounds__(256) void seed_kernel(GridD g, IcSpecD s, StateOut 
       bx = s.b_uniform[0]; by = s.b_uniform[1]; bz = s.b_uniform[2];
       break;
     }
+    case 5: {  // confined_blob (mhd_initial_conditions.hip:190-207): in-plane field zero
+      const bool inside = fabs(xc - s.center[0]) <= s.blob_half && fabs(yc - s.center[1]) <= s.blob_half;
+      rho = inside ? s.rho_in : s.rho_out;
+      p = inside ? s.p_in : s.p_out;
+      bz = s.b_uniform[2];
+      break;
+    }
     default:
       atomicOr(status, 1);
       return;

@@ -60,6 +60,7 @@ struct IcSpecD {
   double r_in, r0, r1, rho_in, rho_out, p_in, p_out, p_core, p_ambient, rho_ambient, u0;
   double ot_rho, ot_p, noise_amplitude;
   unsigned long long noise_seed;
+  double blob_half;
 };
 
 hipError_t upload_quad_rows(const QuadRows& rows);

@@ -1328,7 +1328,7 @@ int qmhd_seed_initial_state(const qmhd_grid* g, const qmhd_ic_spec* spec, double
     const qmhd::GridD d = qmhd::to_grid(*g);
     qmhd::ensure_quad_rows();
     const Scratch s = scratch_of(g, scratch);
-    if (spec->kind < 0 || spec->kind > 4) throw std::invalid_argument("unknown initial condition kind");
+    if (spec->kind < 0 || spec->kind > 5) throw std::invalid_argument("unknown initial condition kind");
     if (!(spec->gamma > 1.0)) throw std::invalid_argument("gamma must be greater than one");
     qmhd::IcSpecD k{};
     k.kind = spec->kind;
@@ -1364,6 +1364,7 @@ int qmhd_seed_initial_state(const qmhd_grid* g, const qmhd_ic_spec* spec, double
     k.ot_p = spec->ot_p;
     k.noise_amplitude = spec->noise_amplitude;
     k.noise_seed = spec->noise_seed;
+    k.blob_half = spec->blob_half;
     int* status = reinterpret_cast<int*>(s.tail());
     qmhd::hip_check(hipMemsetAsync(status, 0, sizeof(int), st(stream)), "memset");
     qmhd::launch_seed(d, k, qmhd::outv(out8), s.plane(0), status, st(stream));

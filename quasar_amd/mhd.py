@@ -27,7 +27,7 @@ STATE_COMPONENTS = ("rho", "mx", "my", "mz", "energy", "bx", "by", "bz")
 def _fma(a: float, b: float, c: float) -> float:
     from fractions import Fraction
     return float(Fraction(a) * Fraction(b) + Fraction(c))
-INITIAL_CONDITIONS = ("brio_wu", "alfven_wave", "orszag_tang", "blast", "rotor")
+INITIAL_CONDITIONS = ("brio_wu", "alfven_wave", "orszag_tang", "blast", "rotor", "confined_blob")
 
 
 def registered_reconstructions() -> List[str]:
@@ -229,6 +229,7 @@ class MhdInitialConditionSpec:
         self.p_core, self.p_ambient, self.rho_ambient, self.u0 = 10.0, 0.1, 1.0, 2.0
         self.ot_rho, self.ot_p = 0.0, 0.0
         self.noise_amplitude, self.noise_seed = 0.0, 0
+        self.blob_half = 0.25
 
     def _c(self) -> _lib.QIcSpec:
         if self.kind not in INITIAL_CONDITIONS:
@@ -248,6 +249,7 @@ class MhdInitialConditionSpec:
         s.p_core, s.p_ambient, s.rho_ambient, s.u0 = self.p_core, self.p_ambient, self.rho_ambient, self.u0
         s.ot_rho, s.ot_p = self.ot_rho, self.ot_p
         s.noise_amplitude, s.noise_seed = self.noise_amplitude, int(self.noise_seed)
+        s.blob_half = self.blob_half
         return s
 
 

@@ -35,7 +35,8 @@ class OrcIcSpec(C.Structure):
                 ("r_in", D), ("r0", D), ("r1", D), ("rho_in", D), ("rho_out", D),
                 ("p_in", D), ("p_out", D), ("p_core", D), ("p_ambient", D),
                 ("rho_ambient", D), ("u0", D),
-                ("ot_rho", D), ("ot_p", D), ("noise_amplitude", D), ("noise_seed", C.c_uint64)]
+                ("ot_rho", D), ("ot_p", D), ("noise_amplitude", D), ("noise_seed", C.c_uint64),
+                ("blob_half", D)]
 
 
 class OrcCounters(C.Structure):
@@ -325,7 +326,7 @@ class Oracle:
 
 
 def ic_spec(kind: str, gamma: float, **kw) -> OrcIcSpec:
-    kinds = {"brio_wu": 0, "alfven_wave": 1, "orszag_tang": 2, "blast": 3, "rotor": 4}
+    kinds = {"brio_wu": 0, "alfven_wave": 1, "orszag_tang": 2, "blast": 3, "rotor": 4, "confined_blob": 5}
     s = OrcIcSpec()
     s.kind = kinds[kind]
     s.gamma = gamma
@@ -336,6 +337,7 @@ def ic_spec(kind: str, gamma: float, **kw) -> OrcIcSpec:
     s.magnetic_velocity_scale = 1.0
     s.r_in, s.r0, s.r1, s.rho_in, s.rho_out = 0.1, 0.1, 0.115, 10.0, 1.0
     s.p_in, s.p_out, s.p_core, s.p_ambient, s.rho_ambient, s.u0 = 1.0, 0.1, 10.0, 0.1, 1.0, 2.0
+    s.blob_half = 0.25
     for k, v in kw.items():
         if k in ("left", "right"):
             getattr(s, k)[:] = v

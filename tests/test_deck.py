@@ -1,0 +1,449 @@
+"""Deck schema, parsing, validation and the CLI's host-side behaviour (no GPU).
+
+The cases follow the reference's deck and CLI tests
+(tests/python/test_mhd_io.py, tests/python/test_mhd_cli.py): invalid decks
+raise ValueError naming the key, scheme and boundary names are checked against
+the solver's registries, and the run loop's time accounting and archive keys
+are checked with a stand-in solver.  Decks for the cylindrical geometry or
+with an enabled background field are refused by this solver."""
+import argparse
+import math
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+import yaml
+
+from quasar_amd import cli, mhd
+from quasar_amd.deck import (BackgroundConfig, BoundaryConfig, Diagnostics, Domain, Initial, MhdDeck, Numerics,
+                             SQRT_MU0, Time, initial_condition_spec, load, parse, parse_side_map)
+
+REPO = Path(__file__).resolve().parents[1]
+
+
+def _domain(**kw):
+    base = dict(nx=16, ny=16, lx_m=1.0, ly_m=1.0)
+    base.update(kw)
+    return Domain(**base)
+
+
+def _numerics(**kw):
+    base = dict(gamma=1.6666667, reconstruction="mp7", riemann="hlld", integrator="ssprk3", ct="fd_ct_christlieb",
+                positivity="troubled_cell", rho_floor=1e-8, p_floor=1e-9, cfl=0.4)
+    base.update(kw)
+    return Numerics(**base)
+
+
+def _deck(**kw):
+    base = dict(domain=_domain(), numerics=_numerics(), initial=Initial(type="orszag_tang"),
+                time=Time(dt_s="auto", steps=5))
+    base.update(kw)
+    return MhdDeck(**base)
+
+
+def _brio(**kw):
+    p = {"interface": 0.5,
+         "left": {"rho": 1.0, "p": 1.0, "vx": 0.0, "vy": 0.0, "vz": 0.0, "bx": 0.75, "by": 1.0, "bz": 0.0},
+         "right": {"rho": 0.125, "p": 0.1, "vx": 0.0, "vy": 0.0, "vz": 0.0, "bx": 0.75, "by": -1.0, "bz": 0.0}}
+    p.update(kw)
+    return p
+
+
+def _full():
+    return {"units": "normalized", "domain": {"nx": 32, "ny": 24, "lx_m": 2.0, "ly_m": 1.0},
+            "geometry": "cartesian",
+            "numerics": {"gamma": 1.6666667, "reconstruction": "mp7", "riemann": "hlld", "integrator": "ssprk3",
+                         "ct": "fd_ct_christlieb", "positivity": "troubled_cell", "rho_floor": 1e-8,
+                         "p_floor": 1e-9, "cfl": 0.4},
+            "initial": {"type": "orszag_tang", "params": {}},
+            "time": {"dt_s": "auto", "steps": 5},
+            "diagnostics": {"output_path": "out.npz", "cadence": 1, "fields": ["rho", "energy"], "divb": True},
+            "boundary": {"fluid": ["periodic"] * 4, "field": ["periodic"] * 4}}
+
+
+# -- loading ---------------------------------------------------------------
+
+def test_parse_full_deck_exposes_fields():
+    d = parse(_full())
+    assert (d.domain.nx, d.domain.ny) == (32, 24)
+    assert math.isclose(d.numerics.gamma, 1.6666667)
+    assert (d.numerics.reconstruction, d.numerics.riemann, d.numerics.integrator) == ("mp7", "hlld", "ssprk3")
+    assert (d.numerics.ct, d.numerics.positivity) == ("fd_ct_christlieb", "troubled_cell")
+    assert d.initial.type == "orszag_tang" and d.time.steps == 5 and d.time.dt_s == "auto"
+    assert d.geometry == "cartesian"
+    assert d.boundary.fluid == ("periodic",) * 4 and d.boundary.field == ("periodic",) * 4
+    assert (d.numerics.arithmetic, d.numerics.indicator) == ("fp64", "none")
+
+
+def test_load_from_yaml_file(tmp_path):
+    path = tmp_path / "deck.yaml"
+    path.write_text(yaml.safe_dump(_full()))
+    d = load(path)
+    assert d.domain.nx == 32 and d.numerics.riemann == "hlld" and d.source_dir == tmp_path.resolve()
+
+
+def test_loader_rejects_duplicate_yaml_keys(tmp_path):
+    path = tmp_path / "dup.yaml"
+    path.write_text("units: normalized\nunits: normalized\n")
+    with pytest.raises(ValueError, match=r"duplicate YAML key 'units'.*line 2"):
+        load(path)
+
+
+def test_minimal_deck_validates():
+    _deck().validate()
+    assert _deck().geometry == "cartesian"
+
+
+def test_internal_tile_boundary_is_not_public_deck_vocabulary():
+    with pytest.raises(ValueError, match="reserved.*distributed"):
+        _deck(boundary=BoundaryConfig(fluid=("internal",) * 4, field=("internal",) * 4)).validate()
+
+
+def test_cylindrical_geometry_and_axis_boundary_are_refused():
+    with pytest.raises(ValueError, match="cylindrical"):
+        _deck(geometry="cylindrical").validate()
+    axis = BoundaryConfig(fluid=("axis", "outflow", "periodic", "periodic"),
+                          field=("axis", "outflow", "periodic", "periodic"))
+    with pytest.raises(ValueError, match="axis"):
+        _deck(boundary=axis).validate()
+    with pytest.raises(ValueError):
+        _deck(geometry="spherical").validate()
+
+
+def test_enabled_background_is_refused_and_disabled_block_is_the_default():
+    with pytest.raises(ValueError, match="background"):
+        _deck(background=BackgroundConfig(enabled=True)).validate()
+    data = _full()
+    data["background_field"] = {"enabled": False, "profile": "whatever", "bx0": 3.0}
+    assert parse(data).background == BackgroundConfig()
+    data["background_field"] = {"enabled": True}
+    with pytest.raises(ValueError, match="background"):
+        parse(data)
+
+
+def test_periodic_boundaries_must_be_paired_and_consistent():
+    one = BoundaryConfig(fluid=("periodic", "outflow", "periodic", "periodic"),
+                         field=("periodic", "outflow", "periodic", "periodic"))
+    with pytest.raises(ValueError, match="both sides"):
+        _deck(boundary=one).validate()
+    mixed = BoundaryConfig(fluid=("periodic",) * 4, field=("outflow", "outflow", "periodic", "periodic"))
+    with pytest.raises(ValueError, match="periodicity must match"):
+        _deck(boundary=mixed).validate()
+
+
+def _small():
+    return {"units": "normalized", "domain": {"nx": 8, "ny": 8, "lx_m": 1.0, "ly_m": 1.0}, "geometry": "cartesian",
+            "numerics": {"gamma": 5.0 / 3.0}, "initial": {"type": "orszag_tang", "params": {}},
+            "time": {"dt_s": "auto", "steps": 2}, "diagnostics": {"cadence": 0, "divb": True},
+            "boundary": {"fluid": "periodic", "field": "periodic"}, "background_field": {"enabled": False}}
+
+
+@pytest.mark.parametrize("section,mutate", [
+    ("top", lambda d: d.__setitem__("geometery", "cylindrical")),
+    ("domain", lambda d: d["domain"].__setitem__("nxx", 16)),
+    ("numerics", lambda d: d["numerics"].__setitem__("cfl_number", 0.2)),
+    ("initial", lambda d: d["initial"].__setitem__("kind", "rotor")),
+    ("time", lambda d: d["time"].__setitem__("step", 10)),
+    ("diagnostics", lambda d: d["diagnostics"].__setitem__("div_b", True)),
+    ("boundary", lambda d: d["boundary"].__setitem__("fields", "wall")),
+    ("background", lambda d: d["background_field"].__setitem__("enable", True)),
+    ("boundary side", lambda d: d["boundary"].__setitem__("fluid", {"x_low": "wall"})),
+])
+def test_unknown_keys_are_rejected_at_every_schema_level(section, mutate):
+    data = _small()
+    mutate(data)
+    with pytest.raises(ValueError, match="unknown"):
+        parse(data)
+
+
+# -- values ----------------------------------------------------------------
+
+@pytest.mark.parametrize("gamma", [0.5, 1.0, float("nan")])
+def test_bad_gamma_rejected(gamma):
+    with pytest.raises(ValueError):
+        _deck(numerics=_numerics(gamma=gamma)).validate()
+
+
+@pytest.mark.parametrize("kw", [dict(nx=0), dict(nx=-4), dict(nx=16.0), dict(nx=True), dict(ny=0),
+                                dict(lx_m=0.0), dict(ly_m=-1.0), dict(lx_m=float("inf")),
+                                dict(lx_m=1e308, origin_x_m=1e308), dict(nx=65537),
+                                dict(lx_m=1e-300, origin_x_m=1.0)])
+def test_bad_domain_rejected(kw):
+    with pytest.raises(ValueError):
+        _deck(domain=_domain(**kw)).validate()
+
+
+@pytest.mark.parametrize("key,value", [("reconstruction", "weno"), ("riemann", "roe"), ("integrator", "rk4"),
+                                       ("ct", "gs05"), ("positivity", "none"), ("arithmetic", "fp16"),
+                                       ("indicator", "cnn")])
+def test_unknown_scheme_names_rejected(key, value):
+    with pytest.raises(ValueError, match=key):
+        _deck(numerics=_numerics(**{key: value})).validate()
+
+
+def test_accepted_reconstructions_equal_registry():
+    for name in mhd.registered_reconstructions():
+        _deck(numerics=_numerics(reconstruction=name)).validate()
+    with pytest.raises(ValueError, match="indicator"):
+        _deck(numerics=_numerics(reconstruction="muscl_minmod", indicator="mlp")).validate()
+    _deck(numerics=_numerics(arithmetic="fp32", indicator="mlp")).validate()
+
+
+@pytest.mark.parametrize("cfl", [0.0, -0.1, 1.5, float("nan")])
+def test_cfl_outside_unit_interval_rejected(cfl):
+    with pytest.raises(ValueError, match="cfl"):
+        _deck(numerics=_numerics(cfl=cfl)).validate()
+
+
+CANONICAL = {"brio_wu": _brio(), "alfven_wave": {"rho": 1.0, "p": 0.1, "b0": 1.0, "wavenumber": 1},
+             "orszag_tang": {"b0": 1.0}, "blast": {"center": [0.5, 0.5]}, "rotor": {},
+             "confined_blob": {"bz": 0.2}}
+
+
+@pytest.mark.parametrize("kind", sorted(CANONICAL))
+def test_canonical_initial_types_accepted(kind):
+    _deck(initial=Initial(type=kind, params=CANONICAL[kind])).validate()
+
+
+@pytest.mark.parametrize("kind,params", [
+    ("not_a_real_ic", {}),
+    ("alfven_wave", {"wavenumber": 1.0}),
+    ("alfven_wave", {"wavenumber": 8}),          # Nyquist of nx = 16
+    ("alfven_wave", {"b0": 0.0}),
+    ("alfven_wave", {"polarization": "linear"}),
+    ("brio_wu", _brio(left={"rho": 1.0, "p": 1.0})),
+    ("brio_wu", _brio(right={"rho": 0.125, "p": 0.1, "vx": 0, "vy": 0, "vz": 0, "bx": 0.5, "by": -1, "bz": 0})),
+    ("brio_wu", _brio(left={"rho": -1.0, "p": 1.0, "vx": 0, "vy": 0, "vz": 0, "bx": 0.75, "by": 1, "bz": 0})),
+    ("blast", {"r_in": 0.0}),
+    ("blast", {"center": [0.5]}),
+    ("rotor", {"r0": 0.2, "r1": 0.1}),
+    ("rotor", {"r0": 0.0}),
+    ("rotor", {"u0": 1e308, "r0": 1e-10, "r1": 1e-9}),
+    ("confined_blob", {"blob_half": -1.0}),
+    ("orszag_tang", {"b1": 1.0}),
+])
+def test_invalid_initial_parameters_rejected(kind, params):
+    with pytest.raises(ValueError):
+        _deck(initial=Initial(type=kind, params=params)).validate()
+
+
+@pytest.mark.parametrize("time", [Time(steps=0), Time(steps=2.0), Time(dt_s="fast"), Time(dt_s=0.0),
+                                  Time(dt_s=-1e-3), Time(t_end=0.0)])
+def test_bad_time_rejected(time):
+    with pytest.raises(ValueError):
+        _deck(time=time).validate()
+
+
+def test_explicit_and_auto_dt_accepted():
+    _deck(time=Time(dt_s=1e-3, steps=3)).validate()
+    _deck(time=Time(dt_s="auto", steps=3, t_end=0.5)).validate()
+
+
+def test_side_map_forms():
+    assert parse_side_map(["periodic", "outflow", "wall", "periodic"], "periodic", "b") == \
+        ("periodic", "outflow", "wall", "periodic")
+    assert parse_side_map("outflow", "periodic", "b") == ("outflow",) * 4
+    assert parse_side_map(None, "periodic", "b") == ("periodic",) * 4
+    assert parse_side_map({"x_lo": "outflow", "x_hi": "outflow"}, "periodic", "b") == \
+        ("outflow", "outflow", "periodic", "periodic")
+    with pytest.raises(ValueError):
+        parse_side_map(["periodic", "outflow"], "periodic", "b")
+
+
+def test_wall_boundaries_round_trip_and_reflecting_is_unknown():
+    data = _small()
+    data["boundary"] = {"fluid": ["wall", "wall", "periodic", "periodic"],
+                        "field": ["wall", "wall", "periodic", "periodic"]}
+    assert parse(data).boundary.fluid == ("wall", "wall", "periodic", "periodic")
+    for side in ("fluid", "field"):
+        bad = BoundaryConfig(**{side: ("outflow", "reflecting", "outflow", "outflow")})
+        with pytest.raises(ValueError):
+            _deck(boundary=bad).validate()
+
+
+@pytest.mark.parametrize("diag", [Diagnostics(output_path=""), Diagnostics(output_path="   "),
+                                  Diagnostics(output_path=123), Diagnostics(cadence=-1), Diagnostics(cadence=1.0),
+                                  Diagnostics(cadence=True), Diagnostics(divb="false"),
+                                  Diagnostics(fields=["rho", "pressure"])])
+def test_bad_diagnostics_rejected(diag):
+    with pytest.raises(ValueError):
+        _deck(diagnostics=diag).validate()
+
+
+def test_parser_does_not_stringify_numeric_output_paths():
+    data = _small()
+    data["diagnostics"] = {"output_path": 123}
+    with pytest.raises(ValueError, match="output_path"):
+        parse(data)
+
+
+# -- lowering to the seed kernel's parameters ---------------------------------
+
+def test_lowering_of_every_initial_type():
+    d = _deck(initial=Initial(type="brio_wu", params=_brio(interface=0.25)))
+    s = initial_condition_spec(d, 4)
+    assert s.kind == "brio_wu" and s.interface == 0.25 and s.grid.nghost == 4
+    assert s.left == [1.0, 1.0, 0.0, 0.0, 0.0, 0.75, 1.0, 0.0] and s.right[0] == 0.125 and s.right[6] == -1.0
+    assert s.scheme_order == 7 and s.magnetic_scale == 1.0
+    s = initial_condition_spec(_deck(initial=Initial(type="alfven_wave", params={"b0": -2.0, "amplitude": 1e-3,
+                                                                                "wavenumber": 2})), 4)
+    assert (s.b0, s.total_b0, s.amplitude, s.wavenumber, s.rho, s.pressure) == (-2.0, -2.0, 1e-3, 2, 1.0, 0.1)
+    s = initial_condition_spec(_deck(initial=Initial(type="orszag_tang", params={"b0": 0.5})), 4)
+    assert s.b_uniform == [0.5, 0.0, 0.0]
+    s = initial_condition_spec(_deck(initial=Initial(type="blast", params={"bx": 0.7, "p_core": 100.0})), 4)
+    assert s.center == [0.0, 0.0] and s.p_core == 100.0 and s.b_uniform == [0.7, 0.0, 0.0] and s.r_in == 0.1
+    s = initial_condition_spec(_deck(initial=Initial(type="rotor", params={"p": 2.0})), 4)
+    assert (s.r0, s.r1, s.rho_in, s.rho_out, s.u0, s.p_ambient) == (0.1, 0.115, 10.0, 1.0, 2.0, 2.0)
+    d = _deck(domain=_domain(lx_m=2.0, origin_x_m=-1.0), initial=Initial(type="confined_blob", params={}))
+    s = initial_condition_spec(d, 4)
+    assert s.center == [0.0, 0.5] and s.blob_half == 0.25 and s.b_uniform == [0.0, 0.0, 0.1]
+    assert (s.rho_in, s.rho_out, s.p_in, s.p_out) == (10.0, 1.0, 1.0, 0.1)
+
+
+def test_si_decks_store_b_over_sqrt_mu0():
+    d = _deck(units="SI", initial=Initial(type="alfven_wave", params={"b0": 1e-3}))
+    s = initial_condition_spec(d, 4)
+    assert s.magnetic_scale == 1.0 / SQRT_MU0 and s.magnetic_velocity_scale == 1.0 / SQRT_MU0
+    assert math.isclose(SQRT_MU0 ** 2, 4e-7 * math.pi, rel_tol=1e-15)
+
+
+# -- CLI on the host -----------------------------------------------------------
+
+def _write(directory: Path, data: dict, name="input.yaml") -> Path:
+    p = directory / name
+    p.write_text(yaml.safe_dump(data))
+    return p
+
+
+def _cli(path: Path, *extra):
+    env = dict(os.environ)
+    env["PYTHONPATH"] = str(REPO) + os.pathsep + env.get("PYTHONPATH", "")
+    return subprocess.run([sys.executable, "-m", "quasar_amd.cli", "run", str(path), *extra], capture_output=True,
+                          text=True, env=env, cwd=str(path.parent))
+
+
+@pytest.mark.parametrize("mutate", [lambda d: d["numerics"].__setitem__("gamma", 0.5),
+                                    lambda d: d["numerics"].__setitem__("reconstruction", "nope"),
+                                    lambda d: d["domain"].__setitem__("nx", 0),
+                                    lambda d: d["initial"].__setitem__("type", "not_a_real_ic")])
+def test_invalid_deck_exits_nonzero_before_stepping(tmp_path, mutate):
+    data = _full()
+    mutate(data)
+    res = _cli(_write(tmp_path, data))
+    assert res.returncode != 0
+    text = (res.stdout + res.stderr).lower()
+    assert "no module named" not in text and "valueerror" in text
+    assert not (tmp_path / "out.npz").exists()
+
+
+def test_distributed_options_are_refused(tmp_path):
+    res = _cli(_write(tmp_path, _full()), "--devices", "0,1")
+    assert res.returncode == 2 and "distributed" in res.stderr
+
+
+def test_output_path_cannot_escape_the_deck_directory(tmp_path):
+    with pytest.raises(ValueError, match="escapes"):
+        cli.confine_output_path(tmp_path, "../out.npz", label="diagnostics.output_path")
+    assert cli.confine_output_path(tmp_path, "sub/out.npz") == (tmp_path / "sub" / "out.npz").resolve()
+    assert (tmp_path / "sub").is_dir()
+
+
+class _Grid:
+    nghost = 0
+
+
+class _StubSolver:
+    """Stands in for MhdSolver2D in the run loop's time accounting."""
+
+    def __init__(self, nx, ny, limits):
+        self._state = np.zeros(nx * ny)
+        self._limits = iter(limits)
+        self.dts, self.divergence_calls = [], 0
+
+    def grid(self):
+        return _Grid()
+
+    def state_component_to_host(self, _name):
+        return self._state.copy()
+
+    def divergence_b_max(self):
+        self.divergence_calls += 1
+        return 0.0
+
+    def conserved_cell_totals(self):
+        return 1.0, 2.0
+
+    def cfl_limit(self):
+        return next(self._limits)
+
+    def step_unchecked(self, dt):
+        self.dts.append(float(dt))
+
+    def step(self, dt):
+        self.dts.append(float(dt))
+
+
+def _stub_deck(t_end, steps=8, kind="blast"):
+    data = _full()
+    data["domain"] = {"nx": 2, "ny": 2, "lx_m": 1.0, "ly_m": 1.0}
+    data["initial"] = {"type": kind, "params": {}}
+    data["time"] = {"dt_s": "auto", "steps": steps, "t_end": t_end}
+    data["diagnostics"] = {"output_path": "out.npz", "cadence": 0, "fields": [], "divb": True}
+    return parse(data)
+
+
+def test_exact_endpoint_is_the_sum_of_actual_solver_steps(tmp_path):
+    solver = _StubSolver(2, 2, [0.1] * 4)
+    cli._run_loop(solver, _stub_deck(0.3), 0.1, True, tmp_path / "out.npz", argparse.Namespace(log_every=0))
+    with np.load(tmp_path / "out.npz", allow_pickle=False) as out:
+        final = float(out["final_time_s"][0])
+        assert int(out["final_step"][0]) == 3
+    assert len(solver.dts) == 3 and solver.dts[-1] == 0.3 - (solver.dts[0] + solver.dts[1])
+    assert math.fsum(solver.dts) == 0.3 and final == 0.3
+
+
+def test_unrepresentable_time_progress_raises(tmp_path):
+    solver = _StubSolver(2, 2, [1.0e16, 0.5])
+    with pytest.raises(RuntimeError, match="too small to advance"):
+        cli._run_loop(solver, _stub_deck(2.0e16), 1.0e16, True, tmp_path / "out.npz",
+                      argparse.Namespace(log_every=0))
+    assert solver.dts == [1.0e16]
+
+
+def test_disabled_divb_performs_no_reductions_and_emits_no_keys(tmp_path):
+    deck = _stub_deck(0.2, steps=2)
+    deck.diagnostics.divb, deck.diagnostics.cadence = False, 1
+    solver = _StubSolver(2, 2, [0.1, 0.1])
+    cli._run_loop(solver, deck, 0.1, True, tmp_path / "out.npz", argparse.Namespace(log_every=1))
+    with np.load(tmp_path / "out.npz", allow_pickle=False) as out:
+        for key in ("divb_linf", "divb_linf_final", "snapshot_divb_linf"):
+            assert key not in out.files
+        assert list(out["snapshot_steps"]) == [1, 2]
+    assert solver.divergence_calls == 0
+
+
+def test_archive_keys_and_cadence_sampling(tmp_path):
+    deck = _stub_deck(None, steps=4, kind="orszag_tang")
+    deck.diagnostics.cadence, deck.diagnostics.fields = 2, ["rho", "bz"]
+    solver = _StubSolver(2, 2, [0.1] * 4)
+    cli._run_loop(solver, deck, 0.1, True, tmp_path / "out.npz", argparse.Namespace(log_every=0))
+    with np.load(tmp_path / "out.npz", allow_pickle=False) as out:
+        keys = set(out.files)
+        for k in ("final_step", "final_time_s", "nx", "ny", "lx_m", "ly_m", "nghost", "units", "geometry",
+                  "gamma", "divb_linf", "divb_linf_final", "snapshot_steps", "snapshot_times_s",
+                  "snapshot_state_rho", "snapshot_state_bz", "snapshot_divb_linf", "mass_initial",
+                  "energy_initial"):
+            assert k in keys, k
+        for name in ("rho", "mx", "my", "mz", "energy", "bx", "by", "bz"):
+            assert out[f"state_{name}"].shape == (2, 2) and out[f"state_{name}_initial"].shape == (2, 2)
+        assert list(out["snapshot_steps"]) == [2, 4] and out["snapshot_state_rho"].shape == (2, 2, 2)
+        # t = 0, the two cadence steps (the last is also the final step).
+        assert out["divb_linf"].shape == (3,)
+        assert float(out["mass_initial"][0]) == 1.0 and str(out["units"][0]) == "normalized"
+    # Explicit dt: the solver's checked step, dt held fixed.
+    solver = _StubSolver(2, 2, [])
+    cli._run_loop(solver, deck, 0.05, False, tmp_path / "fixed.npz", argparse.Namespace(log_every=0))
+    assert solver.dts == [0.05] * 4

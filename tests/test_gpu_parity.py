@@ -54,15 +54,16 @@ def flux_mask(nx, ny, ng):
     return m.ravel()
 
 
-@pytest.mark.parametrize("kind", ["orszag_tang", "brio_wu", "rotor"])
+@pytest.mark.parametrize("kind", ["orszag_tang", "brio_wu", "rotor", "confined_blob"])
 def test_seed_matches_oracle(oracle, kind):
     n, ng = 24, 4
     g = oracle.grid(n, n, ng)
     kw = dict(orszag_tang=dict(b_uniform=[1 / np.sqrt(4 * np.pi), 0, 0], ot_rho=25 / (36 * np.pi),
                                ot_p=5 / (12 * np.pi), noise_amplitude=1e-3, noise_seed=7),
               brio_wu=dict(left=[1, 1, 0, 0, 0, 0.75, 1, 0], right=[0.125, 0.1, 0, 0, 0, 0.75, -1, 0]),
-              rotor=dict(center=[0.5, 0.5], b_uniform=[5 / np.sqrt(4 * np.pi), 0, 0], p_ambient=1.0))[kind]
-    gamma = {"orszag_tang": G53, "brio_wu": 2.0, "rotor": 1.4}[kind]
+              rotor=dict(center=[0.5, 0.5], b_uniform=[5 / np.sqrt(4 * np.pi), 0, 0], p_ambient=1.0),
+              confined_blob=dict(center=[0.5, 0.5], blob_half=0.25, b_uniform=[0, 0, 0.1]))[kind]
+    gamma = {"orszag_tang": G53, "brio_wu": 2.0, "rotor": 1.4, "confined_blob": G53}[kind]
     spec = ic_spec(kind, gamma, **kw)
     st, ref = oracle.seed(g, spec)
     assert st == 0

@@ -250,6 +250,28 @@ def test_orszag_tang_seed_matches_closed_form(oracle):
     np.testing.assert_allclose(e, p0 / (G53 - 1) + kin + 0.5 * (bxc ** 2 + byc ** 2), rtol=1e-14)
 
 
+def test_confined_blob_seed_matches_closed_form(oracle):
+    """Square blob (mhd_initial_conditions.hip:190-207): rho_in/p_in where both
+    |x - cx| and |y - cy| are within blob_half (edges included), rho_out/p_out
+    elsewhere, at rest, with zero in-plane field and a uniform bz."""
+    n, ng, gamma, bz = 16, 4, 1.4, 0.3
+    g = oracle.grid(n, n, ng)
+    spec = ic_spec("confined_blob", gamma, center=[0.5, 0.5], blob_half=0.25, b_uniform=[0, 0, bz],
+                   rho_in=10.0, rho_out=1.0, p_in=2.0, p_out=0.1)
+    st, planes = oracle.seed(g, spec)
+    assert st == 0
+    x = (np.arange(-ng, n + ng) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    inside = (np.abs(X - 0.5) <= 0.25) & (np.abs(Y - 0.5) <= 0.25)
+    assert 0 < inside.sum() < inside.size
+    np.testing.assert_array_equal(planes[0].reshape(X.shape), np.where(inside, 10.0, 1.0))
+    for c in (1, 2, 3, 5, 6):
+        assert not planes[c].any()
+    np.testing.assert_array_equal(planes[7], bz)
+    p = np.where(inside, 2.0, 0.1)
+    np.testing.assert_allclose(planes[4].reshape(X.shape), p / (gamma - 1) + 0.5 * bz * bz, rtol=1e-15)
+
+
 def test_noise_perturbation_is_seeded_and_bounded(oracle):
     n = 12
     _, clean = ot_state(oracle, n)
