@@ -307,7 +307,7 @@ def measured_traffic(kind: str, n: int):
 KERNEL_SYMBOLS = {
     "prep": ("colloc_centred_kernel<true>", "ghost_collocate_kernel", "qmhd::prep_kernel"),
     "reconstruct": ("reconstruct_kernel<double, 7>", "reconstruct_fallback_kernel<double, 7>"),
-    "flux": ("flux_kernel<double, 7, false>",),
+    "flux": ("flux_kernel<double, 7, false, false>",),
     "cell_emf": ("cell_emf_kernel<double, 7, false>",),
     "corner": ("corner_tiled_kernel<double, 7>",),
     "update": ("update_kernel<double>",),
