@@ -319,3 +319,29 @@ def test_positivity_retry_keeps_state_admissible_and_conservative(oracle):
     assert abs(m - m0) <= 1e-13 * m0 and abs(e - e0) <= 1e-13 * e0
     out = s.state()
     assert np.all(interior(out[0], n, n, ng) > 0)
+
+
+def test_brio_wu_reference_fixture_invariants():
+    """The committed oracle run of the Brio-Wu benchmark configuration
+    (tests/golden/make_brio_wu_reference.py) against what the physics fixes:
+    by t = 0.1 no wave has left [0,1], so mass and energy keep their initial
+    totals; a one-dimensional run keeps bx = 0.75 on every face, no z
+    components and equal rows (to the round-off the one-sided ghost-row
+    collocation leaves); the far states keep the initial plateaus."""
+    from pathlib import Path
+    path = Path(__file__).resolve().parent / "golden" / "brio_wu_800x4_mp5_t0.1.npz"
+    ref = np.load(path, allow_pickle=False)
+    rho, mx, my, mz, e, bx, by, bz = ref["interior"]
+    assert ref["interior"].shape == (8, 4, 800)
+    assert float(ref["final_time"][0]) == 0.1 and len(ref["dts"]) == int(ref["steps"][0])
+    assert abs(float(np.sum(ref["dts"])) - 0.1) <= 1e-15
+    # Totals to the round-off of ~3000 steps of flux differences.
+    assert abs(rho.mean() - 0.5625) <= 1e-12
+    e0 = 0.5 * (1.0 + 0.5 * (0.75 ** 2 + 1.0)) + 0.5 * (0.1 + 0.5 * (0.75 ** 2 + 1.0))
+    assert abs(e.mean() - e0) <= 1e-12
+    assert np.max(np.abs(bx - 0.75)) <= 1e-12
+    assert np.max(np.abs(mz)) <= 1e-14 and np.max(np.abs(bz)) <= 1e-14
+    for a in ref["interior"]:
+        assert np.max(np.abs(a - a[0])) <= 1e-12 * max(1.0, np.max(np.abs(a)))
+    assert np.allclose(rho[0, :20], 1.0, atol=1e-12) and np.allclose(rho[0, -20:], 0.125, atol=1e-12)
+    assert float(ref["divb"][0]) <= 1e-10
