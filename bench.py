@@ -88,6 +88,9 @@ def parse():
     ap.add_argument("--sweep-runs", type=int, default=64)
     ap.add_argument("--sweep-seq-steps", type=int, default=30,
                     help="steps per run of the run-by-run comparison of the sweep")
+    ap.add_argument("--problem", choices=("orszag_tang", "rotor"), default="orszag_tang",
+                    help="workload: the Orszag-Tang vortex (periodic) or the MHD rotor (gamma 1.4, disk rho 10 "
+                         "spinning at omega 20 with a taper to r 0.115, Bx 5/sqrt(4 pi), outflow)")
     ap.add_argument("--noise", type=float, default=0.0,
                     help="seeded white-noise amplitude on rho (relative), vx and vy (the 4096^2 deck uses 1e-3)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -115,11 +118,16 @@ def orszag_tang(n: int, noise: float = 0.0):
 
 
 AUTOTUNE_STEPS = 0
+PROBLEM = "orszag_tang"
 
 
-def make_solver(n: int, arithmetic: str = "fp64", indicator: str = "none", noise: float = 0.0):
-    from quasar_amd import mhd
-    spec = orszag_tang(n, noise)
+def make_solver(n: int, arithmetic: str = "fp64", indicator: str = "none", noise: float = 0.0,
+                problem: str = "orszag_tang"):
+    from quasar_amd import batch, mhd
+    if problem == "rotor":
+        spec, sides = batch.problem("rotor", n, noise, 0)
+    else:
+        spec, sides = orszag_tang(n, noise), "periodic"
     cfg = mhd.MhdConfig()
     cfg.grid = spec.grid
     cfg.gamma = spec.gamma
@@ -127,8 +135,8 @@ def make_solver(n: int, arithmetic: str = "fp64", indicator: str = "none", noise
     cfg.arithmetic = arithmetic
     cfg.indicator = indicator
     cfg.cfl = 0.4
-    cfg.boundary.set_fluid_all("periodic")
-    cfg.boundary.set_field_all("periodic")
+    cfg.boundary.set_fluid_all(sides)
+    cfg.boundary.set_field_all(sides)
     solver = mhd.MhdSolver2D(cfg)
     state = mhd.build_initial_state(spec)
     for name, plane in state.items():
@@ -291,9 +299,9 @@ def roofline(solver, n: int):
 
 def measured_traffic(kind: str, n: int):
     """HBM bytes per launch from the committed counter summary, when one exists
-    for this kernel and size."""
+    for this kernel and size (Orszag-Tang runs only)."""
     p = ROOT / "profiles" / f"traffic_ot{n}_mp7.json"
-    if not p.exists():
+    if PROBLEM != "orszag_tang" or not p.exists():
         return None
     try:
         data = json.loads(p.read_text())
@@ -307,7 +315,7 @@ def measured_traffic(kind: str, n: int):
 KERNEL_SYMBOLS = {
     "prep": ("colloc_centred_kernel<true>", "ghost_collocate_kernel", "qmhd::prep_kernel"),
     "reconstruct": ("reconstruct_kernel<double, 7>", "reconstruct_fallback_kernel<double, 7>"),
-    "flux": ("flux_kernel<double, 7, false, false>",),
+    "flux": ("flux_kernel<double, 7, false>",),
     "cell_emf": ("cell_emf_kernel<double, 7, false>",),
     "corner": ("corner_tiled_kernel<double, 7>",),
     "update": ("update_kernel<double>",),
@@ -393,9 +401,10 @@ def cpu_baseline(n: int, budget_s: float):
 
 
 def main():
-    global AUTOTUNE_STEPS
+    global AUTOTUNE_STEPS, PROBLEM
     args = parse()
     AUTOTUNE_STEPS = args.autotune
+    PROBLEM = args.problem
     if args.summarize_counters:
         summary = summarize_counters(args.summarize_counters[0], args.summarize_counters[1], args.n,
                                      args.copy_probe_mib or 1024)
@@ -423,8 +432,8 @@ def main():
     n = args.n
     if args.only_form:
         arith, ind = ("fp32", "none") if args.only_form == "fp32" else ("fp32", "mlp")
-        print(f"bench: seeding Orszag-Tang {n}^2 MP7, form {args.only_form}", file=sys.stderr, flush=True)
-        s = make_solver(n, arith, ind, args.noise)
+        print(f"bench: seeding {PROBLEM} {n}^2 MP7, form {args.only_form}", file=sys.stderr, flush=True)
+        s = make_solver(n, arith, ind, args.noise, PROBLEM)
         if args.warmup > 0:
             s.run(args.warmup)
         s.set_profiling(True)
@@ -434,8 +443,8 @@ def main():
                           "ms_per_step": round(w / args.steps * 1e3, 4), "steps": args.steps, "warmup": args.warmup,
                           "sim_time": st, "kernel_ms": {k: round(v[0], 3) for k, v in times.items()}}), flush=True)
         return
-    print(f"bench: seeding Orszag-Tang {n}^2 MP7 FP64", file=sys.stderr, flush=True)
-    solver = make_solver(n, noise=args.noise)
+    print(f"bench: seeding {PROBLEM} {n}^2 MP7 FP64", file=sys.stderr, flush=True)
+    solver = make_solver(n, noise=args.noise, problem=PROBLEM)
     if args.copy_probe_mib > 0:
         a = torch.ones(args.copy_probe_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
@@ -468,10 +477,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: Orszag-Tang initial condition from its closed form" +
+        "data": f"synthetic: {'Orszag-Tang' if PROBLEM == 'orszag_tang' else 'rotor'} initial condition from "
+                "its closed form" +
                 (f" plus seeded white noise of amplitude {args.noise:g} on rho, vx, vy" if args.noise else ""),
-        "config": {"workload": f"orszag_tang_{n}x{n}_mp7_hlld_ct_ssprk3_fp64", "form": "plain FP64",
-                   "cfl": 0.4, "boundaries": "periodic", "sim_time": sim_t, "noise": args.noise},
+        "config": {"workload": f"{PROBLEM}_{n}x{n}_mp7_hlld_ct_ssprk3_fp64", "form": "plain FP64",
+                   "cfl": 0.4, "boundaries": "periodic" if PROBLEM == "orszag_tang" else "outflow",
+                   "sim_time": sim_t, "noise": args.noise},
         "hbm_frac_step": round(value * 512 / (PEAK_HBM_GBS * 1e9), 5),
         "kernel_ms": shares,
         "roofline": roof,
@@ -485,7 +496,7 @@ def main():
         forms = {}
         for name, arith, ind in (("fp32", "fp32", "none"), ("fp32_learned", "fp32", "mlp")):
             print(f"bench: {name} form", file=sys.stderr, flush=True)
-            s = make_solver(n, arith, ind, args.noise)
+            s = make_solver(n, arith, ind, args.noise, PROBLEM)
             if args.warmup > 0:
                 s.run(args.warmup)
             if ind == "mlp":

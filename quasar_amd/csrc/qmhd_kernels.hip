@@ -352,6 +352,51 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(GridD g, int dir, T ga
   write_faces(out, eq_flag, k, ul, ur);
 }
 
+// Learned form: the characteristic MP states of the faces on the cheap
+// reconstruction's list (one thread per list slot, the grid sized for the
+// longest list); a face whose MP states (or their normal-field update) are not
+// admissible goes on the second list, for the full fallback chain.
+template <class T, int ORDER>
+__global__ __launch_bounds__(256) void reconstruct_list_kernel(GridD g, int dir, T gamma, StateView u,
+                                                               const double* __restrict__ bxc,
+                                                               const double* __restrict__ byc, IfaceView<T> out,
+                                                               int* __restrict__ eq_flag,
+                                                               const unsigned long long* __restrict__ list_count,
+                                                               const unsigned long long* __restrict__ list,
+                                                               unsigned long long* __restrict__ fb_count,
+                                                               unsigned long long* __restrict__ fb_list) {
+  const unsigned long long t = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x;
+  const bool valid = t < *list_count;
+  std::size_t k = 0;
+  St<T> ul, ur;
+  bool need = false;
+  if (valid) {
+    k = static_cast<std::size_t>(list[t]);
+    const int i = static_cast<int>(k % static_cast<std::size_t>(g.pitch)) - g.ng;
+    const int j = static_cast<int>(k / static_cast<std::size_t>(g.pitch)) - g.ng;
+    mp_char_face<T, ORDER>(g, dir, i, j, gamma, u, bxc, byc, ul, ur);
+    bool ok = admissible_state(ul, gamma) && admissible_state(ur, gamma);
+    if (ok) {
+      const T bn = T((dir == 0 ? u.c[5] : u.c[6])[k]);
+      set_normal_b(ul, dir, bn);
+      set_normal_b(ur, dir, bn);
+      ok = admissible_state(ul, gamma) && admissible_state(ur, gamma);
+    }
+    need = !ok;
+  }
+  const unsigned long long mask = __ballot(need);
+  if (mask != 0ull) {
+    const int lane = static_cast<int>(__lane_id());
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(fb_count, static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    if (need) fb_list[base + __popcll(mask & ((1ull << lane) - 1ull))] = k;
+  }
+  if (!valid || need) return;
+  write_faces(out, eq_flag, k, ul, ur);
+}
+
 // The full fallback chain for the faces the MP kernel listed; a small
 // grid-stride launch.
 template <class T, int ORDER>
@@ -397,7 +442,9 @@ __global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir
                                                                 const uint8_t* __restrict__ cell_flags,
                                                                 IfaceView<T> out, int* __restrict__ eq_flag,
                                                                 unsigned long long* __restrict__ fb_count,
-                                                                unsigned long long* __restrict__ fb_list) {
+                                                                unsigned long long* __restrict__ fb_list,
+                                                                FluxView<T> flux, double* __restrict__ rate,
+                                                                int periodic_dir) {
   __shared__ T cells[8][kCheapCells];
   const int di = dir == 0 ? 1 : 0, dj = dir == 0 ? 0 : 1;
   // Tile of cells: columns [ci0, ci0 + cw) x rows [cj0, cj0 + ch).
@@ -460,6 +507,17 @@ __global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir
   }
   if (!valid || need) return;
   write_faces(out, eq_flag, k, ul, ur);
+  // The face's flux from the states just built (faces [0,nx] x [0,ny]); a
+  // periodic high face takes its low partner's flux, stored by the partner.
+  if (i < 0 || j < 0 || i > g.nx || j > g.ny) return;
+  const bool high = dir == 0 ? i == g.nx : j == g.ny;
+  if (periodic_dir && high) return;
+  St<T> f;
+  T smax;
+  cheap_face_flux<T, 5>(ul, ur, dir, gamma, f, smax);
+  store_face(g, dir, flux, rate, k, f, smax);
+  if (periodic_dir && (dir == 0 ? i == 0 : j == 0))
+    store_face(g, dir, flux, rate, dir == 0 ? idx(g, g.nx, j) : idx(g, i, g.ny), f, smax);
 }
 
 // Running total of the faces sent to the plain path.
@@ -526,17 +584,37 @@ __device__ __forceinline__ T face_speed(const St<T>& L, const St<T>& R, int dir,
   return fin(rate) ? rate : Lim<T>::inf();
 }
 
+// The learned form's flux of a face whose two cells the indicator left
+// unflagged: one HLL flux (Davis speeds) at its face-average states, the rate
+// from those states.  flux_face and the cheap-face kernel share it, so both
+// give the same bits.
+template <class T, int ORDER>
+__device__ __forceinline__ void cheap_face_flux(const St<T>& baseL, const St<T>& baseR, int dir, T gamma, St<T>& f,
+                                                T& smax) {
+  smax = face_speed(baseL, baseR, dir, ORDER, gamma);
+  f = riemann<false>(baseL, baseR, dir, gamma, true);
+}
+
+// Face flux and face rate (0.5 * signal speed / spacing) of face ko.
+template <class T>
+__device__ __forceinline__ void store_face(const GridD& g, int dir, const FluxView<T>& flux, double* __restrict__ rate,
+                                           std::size_t ko, const St<T>& f, T smax) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) flux.c[c][ko] = comp(f, c);
+  const double sp = static_cast<double>(smax);
+  const double spacing = dir == 0 ? g.dx : g.dy;
+  rate[ko] = (fin(sp) && sp >= 0.0) ? (sp * 0.5) / spacing : Lim<double>::inf();
+}
+
 // One thread per face in [0,nx] x [0,ny].  Writes the face flux (F_n of the
 // normal field is zero), and the face signal speed max(|v_n|)+max(c_f) over
-// the face average and every Gauss point, scaled by 0.5/spacing.  With
-// `cell_flags` (learned form) a face whose two cells are both unflagged takes
-// one HLL flux at its face-average states (second order across the face, as
-// accurate as needed where the indicator finds the flow smooth) and its rate
-// from those states.
-template <class T, int ORDER, bool LEARNED, bool LF>
+// the face average and every Gauss point, scaled by 0.5/spacing.  (The
+// learned form's unflagged faces take cheap_face_flux instead, in
+// flux_cheap_kernel.)
+template <class T, int ORDER, bool LF>
 __device__ __forceinline__ void flux_face(const GridD& g, int dir, int periodic_dir, T gamma,
                                           const IfaceView<T>& in, const FluxView<T>& flux, double* __restrict__ rate,
-                                          const uint8_t* __restrict__ cell_flags, int i, int j) {
+                                          int i, int j) {
   int si = i, sj = j;
   if (periodic_dir) {
     if (dir == 0 && i == g.nx) si = 0;
@@ -544,22 +622,16 @@ __device__ __forceinline__ void flux_face(const GridD& g, int dir, int periodic_
   }
   const std::size_t ko = idx(g, i, j);
   const std::size_t k = idx(g, si, sj);
-  bool cheap = false;
-  if (LEARNED) cheap = (cell_flags[k] | cell_flags[idx(g, si - (dir == 0), sj - (dir == 1))]) == 0;
   St<T> baseL, baseR;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     set_comp(baseL, c, in.c[c][k]);
     set_comp(baseR, c, in.c[8 + c][k]);
   }
-  const double spacing = dir == 0 ? g.dx : g.dy;
   St<T> f;
   T smax = face_speed(baseL, baseR, dir, ORDER, gamma);
   if (ORDER != 5 && ORDER != 7) {
-    f = riemann<LF>(baseL, baseR, dir, gamma, cheap);
-  } else if (LEARNED && cheap) {
-    // Learned form, smooth face: one HLL flux at the face-average states.
-    f = riemann<LF>(baseL, baseR, dir, gamma, true);
+    f = riemann<LF>(baseL, baseR, dir, gamma);
   } else {
     constexpr int H = ORDER == 5 ? 2 : 3;
     constexpr int NQ = ORDER == 5 ? 3 : 4;
@@ -634,20 +706,61 @@ __device__ __forceinline__ void flux_face(const GridD& g, int dir, int periodic_
 #pragma unroll
     for (int c = 0; c < 8; ++c) set_comp(f, c, same[c] ? first[c] : acc[c]);
   }
-#pragma unroll
-  for (int c = 0; c < 8; ++c) flux.c[c][ko] = comp(f, c);
-  const double sp = static_cast<double>(smax);
-  rate[ko] = (fin(sp) && sp >= 0.0) ? (sp * 0.5) / spacing : Lim<double>::inf();
+  store_face(g, dir, flux, rate, ko, f, smax);
 }
 
-template <class T, int ORDER, bool LEARNED, bool LF>
+template <class T, int ORDER, bool LF>
 __global__ __launch_bounds__(256) void flux_kernel(GridD g, int dir, int periodic_dir, T gamma,
-                                                   IfaceView<T> in, FluxView<T> flux, double* __restrict__ rate,
-                                                   const uint8_t* __restrict__ cell_flags) {
+                                                   IfaceView<T> in, FluxView<T> flux, double* __restrict__ rate) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.nx || j > g.ny) return;
-  flux_face<T, ORDER, LEARNED, LF>(g, dir, periodic_dir, gamma, in, flux, rate, cell_flags, i, j);
+  flux_face<T, ORDER, LF>(g, dir, periodic_dir, gamma, in, flux, rate, i, j);
+}
+
+// Learned form: the flux of the faces on the plain reconstruction's list,
+// after that list's states are written.  The list holds every flagged face of
+// the launch (plain flux) and the unflagged faces whose cheap states were
+// inadmissible (the cheap flux of their plain states); the cheap
+// reconstruction stored every other face's flux.  Ghost-row entries are
+// skipped; a periodic high face takes its low partner's flux.  One thread per
+// list slot, the grid sized for the longest list.
+template <class T, int ORDER>
+__global__ __launch_bounds__(256) void flux_plain_list_kernel(GridD g, int dir, int periodic_dir, T gamma,
+                                                              IfaceView<T> in, FluxView<T> flux,
+                                                              double* __restrict__ rate,
+                                                              const uint8_t* __restrict__ cell_flags,
+                                                              const unsigned long long* __restrict__ fb_count,
+                                                              const unsigned long long* __restrict__ fb_list) {
+  const unsigned long long t = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x;
+  if (t >= *fb_count) return;
+  const std::size_t k = static_cast<std::size_t>(fb_list[t]);
+  const int i = static_cast<int>(k % static_cast<std::size_t>(g.pitch)) - g.ng;
+  const int j = static_cast<int>(k / static_cast<std::size_t>(g.pitch)) - g.ng;
+  if (i < 0 || j < 0 || i > g.nx || j > g.ny) return;
+  if (periodic_dir && (dir == 0 ? i == g.nx : j == g.ny)) return;
+  if ((cell_flags[k] | cell_flags[idx(g, i - (dir == 0), j - (dir == 1))]) != 0) {
+    flux_face<T, ORDER, false>(g, dir, periodic_dir, gamma, in, flux, rate, i, j);
+  } else {
+    St<T> baseL, baseR;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      set_comp(baseL, c, in.c[c][k]);
+      set_comp(baseR, c, in.c[8 + c][k]);
+    }
+    St<T> f;
+    T smax;
+    cheap_face_flux<T, ORDER>(baseL, baseR, dir, gamma, f, smax);
+    store_face(g, dir, flux, rate, k, f, smax);
+  }
+  if (periodic_dir && (dir == 0 ? i == 0 : j == 0)) {
+    // The high face reads this face's states and flags: its flux and rate are
+    // these values.
+    const std::size_t lo = idx(g, i, j), hi = dir == 0 ? idx(g, g.nx, j) : idx(g, i, g.ny);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) flux.c[c][hi] = flux.c[c][lo];
+    rate[hi] = rate[lo];
+  }
 }
 
 // Additive cell CFL rate from the incident face rates, reduced to the maximum.
@@ -816,8 +929,14 @@ __global__ __launch_bounds__(256) void corner_tiled_kernel(GridD g, Flags4 m, Fl
   constexpr int RH = ORDER >= 7 ? 3 : 2;
   constexpr int B = 16;
   constexpr int S = B + NT - 1;  // tile span of cells
+  constexpr int W = NT + 2 * RH;  // faces of a shortcut line
   __shared__ double ce[S][S + 1];
   __shared__ double line[S][B];
+  // Shortcut masks: col_eq[a] bit b = (ce[b+1][a] == ce[b][a]); row_eq[b] bit
+  // a = (ce[b][a+1] == ce[b][a]); eqy_row[ty] bit c = y face (i0+LO-RH+c,
+  // j0+ty) has no jump; eqx_col[tx] bit r = x face (i0+tx, j0+LO-RH+r).
+  __shared__ unsigned col_eq[S], row_eq[S], eqy_row[B], eqx_col[B];
+  static_assert(S <= 32 && B + W - 1 <= 32, "shortcut masks are 32-bit");
   const int i0 = static_cast<int>(blockIdx.x) * B, j0 = static_cast<int>(blockIdx.y) * B;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * B + tx;
   const int i = i0 + tx, j = j0 + ty;
@@ -841,7 +960,44 @@ __global__ __launch_bounds__(256) void corner_tiled_kernel(GridD g, Flags4 m, Fl
     for (int k = 0; k < NT; ++k) e[k] = ce[r][c + k];
     line[r][c] = edge_interp(ORDER, e);
   }
-  __syncthreads();
+  // A shortcut needs the corner's own x or y face to have no jump (offset 0 of
+  // its face lines): when no corner of the block has one, no mask is built.
+  const bool maybe_y = eqy[idx(g, i, j)] != 0, maybe_x = eqx[idx(g, i, j)] != 0;
+  const bool any_shortcut = __syncthreads_or(maybe_y || maybe_x);
+  if (any_shortcut) {
+    // One ballot fills two masks: lanes 0-31 one mask's bits, lanes 32-63 the
+    // next mask's; the block's four waves share the ballots.
+    constexpr int PS = (S + 1) / 2, PB = B / 2;
+    const int lane = tid & 63, half = lane >> 5, bit = lane & 31;
+    for (int task = tid >> 6; task < 2 * PS + 2 * PB; task += (B * B) / 64) {
+      bool pred = false;
+      unsigned* dst;
+      int m0, count;
+      if (task < PS) {
+        const int a = 2 * task + half;
+        pred = a < S && bit + 1 < S && ce[bit + 1][a] == ce[bit][a];
+        dst = col_eq, m0 = 2 * task, count = S;
+      } else if (task < 2 * PS) {
+        const int b = 2 * (task - PS) + half;
+        pred = b < S && bit + 1 < S && ce[b][bit + 1] == ce[b][bit];
+        dst = row_eq, m0 = 2 * (task - PS), count = S;
+      } else if (task < 2 * PS + PB) {
+        const int r = 2 * (task - 2 * PS) + half;
+        pred = bit < B + W - 1 && eqy[idx(g, i0 + LO - RH + bit, j0 + r)] != 0;
+        dst = eqy_row, m0 = 2 * (task - 2 * PS), count = B;
+      } else {
+        const int c = 2 * (task - 2 * PS - PB) + half;
+        pred = bit < B + W - 1 && eqx[idx(g, i0 + c, j0 + LO - RH + bit)] != 0;
+        dst = eqx_col, m0 = 2 * (task - 2 * PS - PB), count = B;
+      }
+      const unsigned long long m = __ballot(pred);
+      if (lane == 0) {
+        dst[m0] = static_cast<unsigned>(m & 0xFFFFFFFFull);
+        if (m0 + 1 < count) dst[m0 + 1] = static_cast<unsigned>(m >> 32);
+      }
+    }
+    __syncthreads();
+  }
   double e[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) e[t] = -static_cast<double>(fx.c[6][idx(g, i, j + LO + t)]);
@@ -854,24 +1010,20 @@ __global__ __launch_bounds__(256) void corner_tiled_kernel(GridD g, Flags4 m, Fl
   const double centered = edge_interp(ORDER, e);
   // One-dimensional shortcuts: every face problem of the stencil line has no
   // jump (faces j+LO-RH .. j+LO+NT-1+RH along the line) and the cell E_z is
-  // constant along the other axis.
-  auto no_jump = [&](const int* eq, int d) {
-    for (int o = LO - RH; o <= LO + NT - 1 + RH; ++o)
-      if (eq[d == 0 ? idx(g, i, j + o) : idx(g, i + o, j)] == 0) return false;
-    return true;
-  };
-  auto constant_along = [&](int axis) {
+  // constant along the other axis (equal to the line's first value, which for
+  // doubles, NaN and signed zeros included, is equality of neighbours).
+  constexpr unsigned WM = (1u << W) - 1u, CM = (1u << (NT - 1)) - 1u;
+  bool x_only = false, y_only = false;
+  if (maybe_y || maybe_x) {
+    bool const_y = true, const_x = true;
+#pragma unroll
     for (int v = 0; v < NT; ++v) {
-      const double ref = axis == 0 ? ce[ty + v][tx] : ce[ty][tx + v];
-      for (int w = 1; w < NT; ++w) {
-        const double c = axis == 0 ? ce[ty + v][tx + w] : ce[ty + w][tx + v];
-        if (c != ref) return false;
-      }
+      const_y = const_y && ((col_eq[tx + v] >> ty) & CM) == CM;
+      const_x = const_x && ((row_eq[ty + v] >> tx) & CM) == CM;
     }
-    return true;
-  };
-  const bool x_only = no_jump(eqy, 1) && constant_along(1);
-  const bool y_only = no_jump(eqx, 0) && constant_along(0);
+    x_only = maybe_y && ((eqy_row[ty] >> tx) & WM) == WM && const_y;
+    y_only = maybe_x && ((eqx_col[tx] >> ty) & WM) == WM && const_x;
+  }
   double r;
   if (x_only && !y_only)
     r = ex;
@@ -1524,8 +1676,10 @@ template <class T>
 void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma, const Flags4& flags,
                                 const StateView& u, const double* bxc, const double* byc, const uint8_t* cell_flags,
                                 const IfaceView<T>& out, int* eq, unsigned long long* fb_count,
-                                unsigned long long* fb_list, unsigned long long* plain_total, hipStream_t stream,
-                                int list_blocks) {
+                                unsigned long long* fb_list, unsigned long long* fb2_count,
+                                unsigned long long* fb2_list, unsigned long long* plain_total,
+                                const FluxView<T>& flux, double* rate, hipStream_t stream, int list_blocks) {
+  const int periodic = dir == 0 ? (flags.side[0] == 0 && flags.side[1] == 0) : (flags.side[2] == 0 && flags.side[3] == 0);
   const int lx = dir == 0 ? g.nx + 1 : g.nx + 2 * g.ng;
   const int ly = dir == 0 ? g.ny + 2 * g.ng : g.ny + 1;
   const int slo = dir == 0 ? flags.side[0] : flags.side[2];
@@ -1535,50 +1689,72 @@ void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma
   // the device at the MP kernel's occupancy.
   const dim3 list_grid(static_cast<unsigned>(list_blocks)), list_block(256);
   hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb_count);
+  hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb2_count);
   const dim3 cb(kCheapTx, kCheapTy);
   hipLaunchKernelGGL((cheap_reconstruct_kernel<T>), blocks2d(lx, ly, cb), cb, 0, stream, g, dir, gamma_t, u, bxc, byc,
-                     cell_flags, out, eq, fb_count, fb_list);
-  if (order == 5)
+                     cell_flags, out, eq, fb_count, fb_list, flux, rate, periodic);
+  // The list holds at most the launch's faces.
+  const dim3 slot_grid(static_cast<unsigned>((static_cast<std::size_t>(lx) * ly + 255) / 256));
+  if (order == 5) {
+    hipLaunchKernelGGL((reconstruct_list_kernel<T, 5>), slot_grid, list_block, 0, stream, g, dir, gamma_t, u, bxc, byc,
+                       out, eq, fb_count, fb_list, fb2_count, fb2_list);
     hipLaunchKernelGGL((reconstruct_fallback_kernel<T, 5>), list_grid, list_block, 0, stream, g, dir, gamma_t, slo, shi,
-                       u, bxc, byc, out, eq, fb_count, fb_list);
-  else
+                       u, bxc, byc, out, eq, fb2_count, fb2_list);
+  } else {
+    hipLaunchKernelGGL((reconstruct_list_kernel<T, 7>), slot_grid, list_block, 0, stream, g, dir, gamma_t, u, bxc, byc,
+                       out, eq, fb_count, fb_list, fb2_count, fb2_list);
     hipLaunchKernelGGL((reconstruct_fallback_kernel<T, 7>), list_grid, list_block, 0, stream, g, dir, gamma_t, slo, shi,
-                       u, bxc, byc, out, eq, fb_count, fb_list);
+                       u, bxc, byc, out, eq, fb2_count, fb2_list);
+  }
   if (plain_total) hipLaunchKernelGGL(plain_faces_accumulate_kernel, dim3(1), dim3(64), 0, stream, fb_count, plain_total);
 }
 
 template <class T>
 void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4& flags, double gamma,
                  const IfaceView<T>& in, const FluxView<T>& flux, double* rate, hipStream_t stream,
-                 const uint8_t* cell_flags, dim3 b) {
+                 dim3 b) {
   const T gm = T(gamma);
   const dim3 grid = blocks2d(g.nx + 1, g.ny + 1, b);
   const int periodic = dir == 0 ? (flags.side[0] == 0 && flags.side[1] == 0) : (flags.side[2] == 0 && flags.side[3] == 0);
-  if (cell_flags && (hll_only || (order != 5 && order != 7)))
-    throw std::invalid_argument("flux launch: the learned form runs the MP5/MP7 HLLD flux");
   auto one_lane = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, grid, b, 0, stream, g, dir, periodic, gm, in, flux, rate, cell_flags);
+    hipLaunchKernelGGL(kernel, grid, b, 0, stream, g, dir, periodic, gm, in, flux, rate);
   };
   switch (order) {
     case 1:
-      hll_only ? one_lane(flux_kernel<T, 1, false, true>) : one_lane(flux_kernel<T, 1, false, false>);
+      hll_only ? one_lane(flux_kernel<T, 1, true>) : one_lane(flux_kernel<T, 1, false>);
       break;
     case 2:
-      hll_only ? one_lane(flux_kernel<T, 2, false, true>) : one_lane(flux_kernel<T, 2, false, false>);
+      hll_only ? one_lane(flux_kernel<T, 2, true>) : one_lane(flux_kernel<T, 2, false>);
       break;
     case 5:
-      if (cell_flags)
-        one_lane(flux_kernel<T, 5, true, false>);
-      else
-        hll_only ? one_lane(flux_kernel<T, 5, false, true>) : one_lane(flux_kernel<T, 5, false, false>);
+      hll_only ? one_lane(flux_kernel<T, 5, true>) : one_lane(flux_kernel<T, 5, false>);
       break;
     default:
-      if (cell_flags)
-        one_lane(flux_kernel<T, 7, true, false>);
-      else
-        hll_only ? one_lane(flux_kernel<T, 7, false, true>) : one_lane(flux_kernel<T, 7, false, false>);
+      hll_only ? one_lane(flux_kernel<T, 7, true>) : one_lane(flux_kernel<T, 7, false>);
       break;
   }
+}
+
+template <class T>
+void launch_flux_learned(const GridD& g, int dir, int order, const Flags4& flags, double gamma,
+                         const IfaceView<T>& in, const FluxView<T>& flux, double* rate, const uint8_t* cell_flags,
+                         const unsigned long long* fb_count, const unsigned long long* fb_list, hipStream_t stream,
+                         dim3 b, int list_blocks) {
+  if (order != 5 && order != 7) throw std::invalid_argument("flux launch: the learned form runs MP5/MP7");
+  const T gm = T(gamma);
+  const int periodic = dir == 0 ? (flags.side[0] == 0 && flags.side[1] == 0) : (flags.side[2] == 0 && flags.side[3] == 0);
+  // The list holds at most the reconstruction launch's faces.
+  const std::size_t max_list = dir == 0 ? static_cast<std::size_t>(g.nx + 1) * (g.ny + 2 * g.ng)
+                                        : static_cast<std::size_t>(g.nx + 2 * g.ng) * (g.ny + 1);
+  const dim3 list_grid(static_cast<unsigned>((max_list + 255) / 256)), list_block(256);
+  (void)list_blocks;
+  (void)b;
+  if (order == 5)
+    hipLaunchKernelGGL((flux_plain_list_kernel<T, 5>), list_grid, list_block, 0, stream, g, dir, periodic, gm, in,
+                       flux, rate, cell_flags, fb_count, fb_list);
+  else
+    hipLaunchKernelGGL((flux_plain_list_kernel<T, 7>), list_grid, list_block, 0, stream, g, dir, periodic, gm, in,
+                       flux, rate, cell_flags, fb_count, fb_list);
 }
 
 void launch_cfl_reduce(const GridD& g, const double* rate_x, const double* rate_y, unsigned long long* max_rate,
@@ -1701,7 +1877,7 @@ hipError_t upload_quad_rows(const QuadRows& rows) {
                                       const double*, const double*, const IfaceView<T>&, int*, unsigned long long*, \
                                       unsigned long long*, hipStream_t);                                          \
   template void launch_flux<T>(const GridD&, int, int, int, const Flags4&, double, const IfaceView<T>&,            \
-                               const FluxView<T>&, double*, hipStream_t, const uint8_t*, dim3);                   \
+                               const FluxView<T>&, double*, hipStream_t, dim3);                                   \
   template void launch_cell_emf<T>(const GridD&, int, const Flags4&, double, const StateView&, const double*,     \
                                    const double*, double*, hipStream_t, const uint8_t*);                          \
   template void launch_corner<T>(const GridD&, int, const Flags4&, const FluxView<T>&, const FluxView<T>&,        \
@@ -1710,10 +1886,14 @@ hipError_t upload_quad_rows(const QuadRows& rows) {
                                  const StateView&, const FluxView<T>&, const FluxView<T>&, const double*,         \
                                  const StateOut&, hipStream_t, dim3);
 #define QMHD_INSTANTIATE_LEARNED(T)                                                                                \
+  template void launch_flux_learned<T>(const GridD&, int, int, const Flags4&, double, const IfaceView<T>&,         \
+                                       const FluxView<T>&, double*, const uint8_t*, const unsigned long long*,      \
+                                       const unsigned long long*, hipStream_t, dim3, int);                         \
   template void launch_reconstruct_learned<T>(const GridD&, int, int, double, const Flags4&, const StateView&,     \
                                               const double*, const double*, const uint8_t*, const IfaceView<T>&,  \
                                               int*, unsigned long long*, unsigned long long*, unsigned long long*, \
-                                              hipStream_t, int);
+                                              unsigned long long*, unsigned long long*, const FluxView<T>&,       \
+                                              double*, hipStream_t, int);
 QMHD_INSTANTIATE_LEARNED(double)
 QMHD_INSTANTIATE_LEARNED(float)
 #undef QMHD_INSTANTIATE_LEARNED

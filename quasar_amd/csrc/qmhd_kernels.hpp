@@ -81,16 +81,24 @@ void launch_reconstruct(const GridD& g, int dir, int order, double gamma, const 
 // Learned form (orders 5 and 7): cheap faces where the indicator left both
 // cells unflagged, the plain path over the list of the others; adds the
 // listed count to *plain_total when given.
+// Learned form: the flux of the faces on the plain reconstruction's list (the
+// cheap reconstruction stores every other face's flux and rate itself).
+template <class T>
+void launch_flux_learned(const GridD& g, int dir, int order, const Flags4& flags, double gamma,
+                         const IfaceView<T>& in, const FluxView<T>& flux, double* rate, const uint8_t* cell_flags,
+                         const unsigned long long* fb_count, const unsigned long long* fb_list, hipStream_t stream,
+                         dim3 b, int list_blocks);
 template <class T>
 void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma, const Flags4& flags,
                                 const StateView& u, const double* bxc, const double* byc, const uint8_t* cell_flags,
                                 const IfaceView<T>& out, int* eq, unsigned long long* fb_count,
-                                unsigned long long* fb_list, unsigned long long* plain_total, hipStream_t stream,
-                                int list_blocks = 1024);
+                                unsigned long long* fb_list, unsigned long long* fb2_count,
+                                unsigned long long* fb2_list, unsigned long long* plain_total,
+                                const FluxView<T>& flux, double* rate, hipStream_t stream, int list_blocks = 1024);
 template <class T>
 void launch_flux(const GridD& g, int dir, int order, int hll_only, const Flags4& flags, double gamma,
                  const IfaceView<T>& in, const FluxView<T>& flux, double* rate, hipStream_t stream,
-                 const uint8_t* cell_flags = nullptr, dim3 block = dim3(64, 4));
+                 dim3 block = dim3(64, 4));
 void launch_cfl_reduce(const GridD& g, const double* rate_x, const double* rate_y, unsigned long long* max_rate,
                        hipStream_t stream);
 template <class T>

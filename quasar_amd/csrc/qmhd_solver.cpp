@@ -359,6 +359,10 @@ class Solver {
     fb_list_ = DBuf(n);
     fb_count_ = DBuf(1);
     if (learned_) {
+      fb2_list_ = DBuf(n);
+      fb2_count_ = DBuf(1);
+    }
+    if (learned_) {
       mlp_params_ = DBuf((mlp::kParams + 1) / 2);
       mlp_packed_ = DBuf((mlp::kPackTotal + 3) / 4);
       cell_flags_ = DBuf((n + 7) / 8);
@@ -507,11 +511,12 @@ class Solver {
   }
 
   template <class T>
-  void issue_reconstruct(Register& r, int dir, int order, double* const* ip, int* eq) {
+  void issue_reconstruct(Register& r, int dir, int order, double* const* ip, int* eq, double* const* fp,
+                         double* rate) {
     if (learned_ && order == order_)
       launch_reconstruct_learned<T>(g_, dir, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, flags(),
-                                    iface_as<T>(ip), eq, fb_count(), fb_list(), plain_total(), stream_,
-                                    tuning_.list_blocks);
+                                    iface_as<T>(ip), eq, fb_count(), fb_list(), fb2_count(), fb2_list(), plain_total(),
+                                    flux_as<T>(fp), rate, stream_, tuning_.list_blocks);
     else
       launch_reconstruct<T>(g_, dir, order, gamma_, field_, r.view(), r.bxc.p, r.byc.p, iface_as<T>(ip), eq,
                             fb_count(), fb_list(), stream_);
@@ -521,17 +526,21 @@ class Solver {
   void issue_faces_as(Register& r, int order, int hll, double* const* ip, double* const* fxp, double* const* fyp) {
     const uint8_t* cf = learned_ && order == order_ ? flags() : nullptr;
     clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_,
-                 [&] { issue_reconstruct<T>(r, 0, order, ip, reinterpret_cast<int*>(eqx_.p)); });
-    clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
-      launch_flux<T>(g_, 0, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fxp), rate_x_.p, stream_, cf,
-                     tuning_.flux);
-    });
+                 [&] { issue_reconstruct<T>(r, 0, order, ip, reinterpret_cast<int*>(eqx_.p), fxp, rate_x_.p); });
+    auto flux = [&](int dir, double* const* fp, double* rate) {
+      clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
+        if (cf)
+          launch_flux_learned<T>(g_, dir, order, field_, gamma_, iface_as<T>(ip), flux_as<T>(fp), rate, cf,
+                                 fb_count(), fb_list(), stream_, tuning_.flux, tuning_.list_blocks);
+        else
+          launch_flux<T>(g_, dir, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fp), rate, stream_,
+                         tuning_.flux);
+      });
+    };
+    flux(0, fxp, rate_x_.p);
     clock_.timed(QMHD_KERNEL_RECONSTRUCT, stream_,
-                 [&] { issue_reconstruct<T>(r, 1, order, ip, reinterpret_cast<int*>(eqy_.p)); });
-    clock_.timed(QMHD_KERNEL_FLUX, stream_, [&] {
-      launch_flux<T>(g_, 1, order, hll, field_, gamma_, iface_as<T>(ip), flux_as<T>(fyp), rate_y_.p, stream_, cf,
-                     tuning_.flux);
-    });
+                 [&] { issue_reconstruct<T>(r, 1, order, ip, reinterpret_cast<int*>(eqy_.p), fyp, rate_y_.p); });
+    flux(1, fyp, rate_y_.p);
   }
 
   // Cell E_z, corner E_z and the fused update of one stage.
@@ -966,8 +975,10 @@ class Solver {
   DBuf iface_[16], fx_[8], fy_[8];
   DBuf rate_x_, rate_y_, eqx_, eqy_, cell_ez_, ez_;
   // Faces whose MP states need the fallback chain (list + count).
-  DBuf fb_list_, fb_count_;
+  DBuf fb_list_, fb_count_, fb2_list_, fb2_count_;
   unsigned long long* fb_list() { return reinterpret_cast<unsigned long long*>(fb_list_.p); }
+  unsigned long long* fb2_list() { return reinterpret_cast<unsigned long long*>(fb2_list_.p); }
+  unsigned long long* fb2_count() { return reinterpret_cast<unsigned long long*>(fb2_count_.p); }
   unsigned long long* fb_count() { return reinterpret_cast<unsigned long long*>(fb_count_.p); }
   StepStatus* status_{nullptr};
   StepStatus* host_status_{nullptr};
