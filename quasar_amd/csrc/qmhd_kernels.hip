@@ -432,6 +432,7 @@ __global__ void counter_reset_kernel(unsigned long long* __restrict__ c) {
 // (64 x 4 faces) stages the cells its stencils read in LDS: 69 x 4 for x faces,
 // 64 x 9 for y faces.
 constexpr int kCheapTx = 64, kCheapTy = 4;
+static_assert(kCheapTx * kCheapTy == 256, "the cheap kernel's list reservation assumes four waves per block");
 constexpr int kCheapCells = (kCheapTx + 5) * kCheapTy > kCheapTx * (kCheapTy + 5) ? (kCheapTx + 5) * kCheapTy
                                                                                     : kCheapTx * (kCheapTy + 5);
 
@@ -496,14 +497,25 @@ __global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir
       need = !(admissible_state(ul, gamma) && admissible_state(ur, gamma));
     }
   }
-  const unsigned long long mask = __ballot(need);
-  if (mask != 0ull) {
-    const int lane = static_cast<int>(__lane_id());
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    unsigned long long fb_base = 0;
-    if (lane == leader) fb_base = atomicAdd(fb_count, static_cast<unsigned long long>(__popcll(mask)));
-    fb_base = __shfl(fb_base, leader, 64);
-    if (need) fb_list[fb_base + __popcll(mask & ((1ull << lane) - 1ull))] = k;
+  // One list reservation per block (where most faces are flagged, an atomic
+  // per wave on the one counter serialises the launch).
+  {
+    __shared__ unsigned wave_count[4];
+    __shared__ unsigned long long block_base;
+    const unsigned long long mask = __ballot(need);
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane == 0) wave_count[wave] = static_cast<unsigned>(__popcll(mask));
+    __syncthreads();
+    if (tid == 0) {
+      const unsigned total = wave_count[0] + wave_count[1] + wave_count[2] + wave_count[3];
+      block_base = total ? atomicAdd(fb_count, static_cast<unsigned long long>(total)) : 0ull;
+    }
+    __syncthreads();
+    if (need) {
+      unsigned long long at = block_base + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull)));
+      for (int w = 0; w < wave; ++w) at += wave_count[w];
+      fb_list[at] = k;
+    }
   }
   if (!valid || need) return;
   write_faces(out, eq_flag, k, ul, ur);
