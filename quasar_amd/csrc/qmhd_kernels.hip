@@ -431,6 +431,13 @@ __global__ void counter_reset_kernel(unsigned long long* __restrict__ c) {
 // for the plain path (characteristic MP with its fallback chain).  The block
 // (64 x 4 faces) stages the cells its stencils read in LDS: 69 x 4 for x faces,
 // 64 x 9 for y faces.
+template <class T, int ORDER>
+__device__ __forceinline__ void cheap_face_flux(const St<T>& baseL, const St<T>& baseR, int dir, T gamma, St<T>& f,
+                                                T& smax);
+template <class T>
+__device__ __forceinline__ void store_face(const GridD& g, int dir, const FluxView<T>& flux, double* __restrict__ rate,
+                                           std::size_t ko, const St<T>& f, T smax);
+
 constexpr int kCheapTx = 64, kCheapTy = 4;
 static_assert(kCheapTx * kCheapTy == 256, "the cheap kernel's list reservation assumes four waves per block");
 constexpr int kCheapCells = (kCheapTx + 5) * kCheapTy > kCheapTx * (kCheapTy + 5) ? (kCheapTx + 5) * kCheapTy
