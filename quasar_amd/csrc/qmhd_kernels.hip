@@ -525,7 +525,25 @@ __global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir
     }
   }
   if (!valid || need) return;
-  write_faces(out, eq_flag, k, ul, ur);
+  // This face's states are read back only by the transverse Gauss recovery of
+  // a flagged face at most 3 faces away along the face line (the plain flux).
+  // With no flagged face that close it stores only its L == R flag; the two
+  // state planes keep stale values nothing reads.
+  bool read_back = false;
+#pragma unroll
+  for (int d = -3; d <= 3; ++d) {
+    const int ii = i + (dir == 1 ? d : 0), jj = j + (dir == 0 ? d : 0);
+    if (ii < -g.ng || jj < -g.ng || ii >= g.nx + g.ng || jj >= g.ny + g.ng) continue;
+    read_back = read_back || (cell_flags[idx(g, ii, jj)] | cell_flags[idx(g, ii - di, jj - dj)]) != 0;
+  }
+  if (read_back) {
+    write_faces(out, eq_flag, k, ul, ur);
+  } else {
+    bool eq = true;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) eq = eq && (comp(ul, c) == comp(ur, c));
+    eq_flag[k] = eq ? 1 : 0;
+  }
   // The face's flux from the states just built (faces [0,nx] x [0,ny]); a
   // periodic high face takes its low partner's flux, stored by the partner.
   if (i < 0 || j < 0 || i > g.nx || j > g.ny) return;
