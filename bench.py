@@ -61,8 +61,8 @@ UNIT_COST = {
     # E_z planes in, 8 planes out
     "update": {"bytes": 8 * (8 + 8 + 16 + 1 + 8), "flops": 90},
     # two correctly rounded 8-tap collocations, theta and checks; 8 candidate +
-    # 8 base planes in, 2 collocated field planes out
-    "prep": {"bytes": 8 * (8 + 8 + 2), "flops": 1000},
+    # 8 base + 2 base collocated field planes in, 2 collocated field planes out
+    "prep": {"bytes": 8 * (8 + 8 + 2 + 2), "flops": 1000},
 }
 NG = 4
 
