@@ -220,10 +220,10 @@ def test_indicator_kernel_matches_network_on_features(oracle):
         assert set(np.unique(f).tolist()) <= {0, 1}
 
 
-def learned_pair(oracle, n, arithmetic, threshold, params=None, kind="orszag_tang", steps=3):
-    from quasar_amd import mhd
-    ng = 4
-    g = oracle.grid(n, n, ng)
+def seeded_planes(oracle, n, kind):
+    """gamma, boundary kind and the oracle-seeded state planes of a square
+    n x n MP7 test problem."""
+    g = oracle.grid(n, n, 4)
     if kind == "orszag_tang":
         gamma, sides = 5 / 3, "periodic"
         spec = ic_spec("orszag_tang", gamma, b_uniform=[1 / np.sqrt(4 * np.pi), 0, 0], ot_rho=25 / (36 * np.pi),
@@ -233,6 +233,13 @@ def learned_pair(oracle, n, arithmetic, threshold, params=None, kind="orszag_tan
         spec = ic_spec("rotor", gamma, center=[0.5, 0.5], b_uniform=[5 / np.sqrt(4 * np.pi), 0, 0], p_ambient=1.0)
     st, planes = oracle.seed(g, spec)
     assert st == 0
+    return gamma, sides, planes
+
+
+def learned_pair(oracle, n, arithmetic, threshold, params=None, kind="orszag_tang", steps=3):
+    from quasar_amd import mhd
+    ng = 4
+    gamma, sides, planes = seeded_planes(oracle, n, kind)
     sols = []
     for ind in ("none", "mlp"):
         cfg = mhd.MhdConfig()
@@ -318,3 +325,28 @@ def test_trained_indicator_gains_over_both_extremes(oracle):
     assert err_learned is not None and err_learned <= 1e-3
     assert frac_learned <= 0.6
     assert err_cheap is None or err_cheap >= 3 * err_learned
+
+
+@pytest.mark.skipif(not WEIGHTS.exists(), reason="trained indicator weights not present")
+def test_learned_step_reads_no_stale_interface_states(oracle):
+    """A cheap face stores its interface states only where a flagged face's
+    transverse stencil reads them; elsewhere the interface buffer keeps
+    whatever it held.  Rotor 96^2 with the trained network (flagged and
+    unflagged faces side by side): three learned steps give the same bits
+    from a fresh solver and from one whose buffer holds the faces of six
+    earlier steps."""
+    from quasar_amd import learned as lrn
+    from quasar_amd import mhd
+    params, thr, _ = lrn.load(WEIGHTS)
+    n = 96
+    _, fresh = learned_pair(oracle, n, "fp32", thr, params, kind="rotor")
+    _, used = learned_pair(oracle, n, "fp32", thr, params, kind="rotor")
+    _, _, planes = seeded_planes(oracle, n, "rotor")
+    assert used.run(6)[0] == 6
+    for k, nm in enumerate(mhd.STATE_COMPONENTS):
+        used.seed_state(nm, planes[k])
+    assert fresh.run(3)[0] == 3 and used.run(3)[0] == 3
+    pf, total = fresh.plain_faces()
+    assert 0 < pf < total
+    for nm in mhd.STATE_COMPONENTS + ("bx_face", "by_face"):
+        np.testing.assert_array_equal(used.state_component_to_host(nm), fresh.state_component_to_host(nm))
