@@ -646,8 +646,9 @@ __device__ __forceinline__ void store_face(const GridD& g, int dir, const FluxVi
 // One thread per face in [0,nx] x [0,ny].  Writes the face flux (F_n of the
 // normal field is zero), and the face signal speed max(|v_n|)+max(c_f) over
 // the face average and every Gauss point, scaled by 0.5/spacing.  (The
-// learned form's unflagged faces take cheap_face_flux instead, in
-// flux_cheap_kernel.)
+// learned form's unflagged faces take cheap_face_flux instead, in the cheap
+// reconstruction or, for faces whose cheap states were inadmissible, in
+// flux_plain_list_kernel.)
 template <class T, int ORDER, bool LF>
 __device__ __forceinline__ void flux_face(const GridD& g, int dir, int periodic_dir, T gamma,
                                           const IfaceView<T>& in, const FluxView<T>& flux, double* __restrict__ rate,
