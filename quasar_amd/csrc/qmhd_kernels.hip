@@ -423,6 +423,14 @@ __global__ void counter_reset_kernel(unsigned long long* __restrict__ c) {
   if (threadIdx.x == 0) c[0] = 0ull;
 }
 
+// Both list counters of a learned reconstruction launch, in one launch.
+__global__ void counter_pair_reset_kernel(unsigned long long* __restrict__ a, unsigned long long* __restrict__ b) {
+  if (threadIdx.x == 0) {
+    a[0] = 0ull;
+    b[0] = 0ull;
+  }
+}
+
 // Learned form, one thread per face: a face whose two cells the indicator
 // left unflagged takes the linear fifth-order upwind-biased face values (the
 // unlimited MP5 interpolant, include/quasar/numerics/mp_limiter.hpp:182-191)
@@ -1726,8 +1734,7 @@ void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma
   // The plain list can hold a large share of the faces: enough blocks to fill
   // the device at the MP kernel's occupancy.
   const dim3 list_grid(static_cast<unsigned>(list_blocks)), list_block(256);
-  hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb_count);
-  hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, stream, fb2_count);
+  hipLaunchKernelGGL(counter_pair_reset_kernel, dim3(1), dim3(64), 0, stream, fb_count, fb2_count);
   const dim3 cb(kCheapTx, kCheapTy);
   hipLaunchKernelGGL((cheap_reconstruct_kernel<T>), blocks2d(lx, ly, cb), cb, 0, stream, g, dir, gamma_t, u, bxc, byc,
                      cell_flags, out, eq, fb_count, fb_list, flux, rate, periodic);
