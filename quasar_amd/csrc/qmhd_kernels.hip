@@ -364,8 +364,11 @@ __global__ __launch_bounds__(256) void reconstruct_list_kernel(GridD g, int dir,
                                                                const unsigned long long* __restrict__ list_count,
                                                                const unsigned long long* __restrict__ list,
                                                                unsigned long long* __restrict__ fb_count,
-                                                               unsigned long long* __restrict__ fb_list) {
+                                                               unsigned long long* __restrict__ fb_list,
+                                                               unsigned long long* __restrict__ plain_total) {
   const unsigned long long t = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x;
+  // Running total of the faces sent to the plain path (one thread per launch).
+  if (plain_total && t == 0) plain_total[0] += *list_count;
   const bool valid = t < *list_count;
   std::size_t k = 0;
   St<T> ul, ur;
@@ -565,11 +568,6 @@ __global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir
     store_face(g, dir, flux, rate, dir == 0 ? idx(g, g.nx, j) : idx(g, i, g.ny), f, smax);
 }
 
-// Running total of the faces sent to the plain path.
-__global__ void plain_faces_accumulate_kernel(const unsigned long long* __restrict__ fb_count,
-                                              unsigned long long* __restrict__ total) {
-  if (threadIdx.x == 0) total[0] += fb_count[0];
-}
 
 // ---------------------------------------------------------------------------
 // Face flux with transverse Gauss quadrature and the CFL face rate.
@@ -1742,16 +1740,15 @@ void launch_reconstruct_learned(const GridD& g, int dir, int order, double gamma
   const dim3 slot_grid(static_cast<unsigned>((static_cast<std::size_t>(lx) * ly + 255) / 256));
   if (order == 5) {
     hipLaunchKernelGGL((reconstruct_list_kernel<T, 5>), slot_grid, list_block, 0, stream, g, dir, gamma_t, u, bxc, byc,
-                       out, eq, fb_count, fb_list, fb2_count, fb2_list);
+                       out, eq, fb_count, fb_list, fb2_count, fb2_list, plain_total);
     hipLaunchKernelGGL((reconstruct_fallback_kernel<T, 5>), list_grid, list_block, 0, stream, g, dir, gamma_t, slo, shi,
                        u, bxc, byc, out, eq, fb2_count, fb2_list);
   } else {
     hipLaunchKernelGGL((reconstruct_list_kernel<T, 7>), slot_grid, list_block, 0, stream, g, dir, gamma_t, u, bxc, byc,
-                       out, eq, fb_count, fb_list, fb2_count, fb2_list);
+                       out, eq, fb_count, fb_list, fb2_count, fb2_list, plain_total);
     hipLaunchKernelGGL((reconstruct_fallback_kernel<T, 7>), list_grid, list_block, 0, stream, g, dir, gamma_t, slo, shi,
                        u, bxc, byc, out, eq, fb2_count, fb2_list);
   }
-  if (plain_total) hipLaunchKernelGGL(plain_faces_accumulate_kernel, dim3(1), dim3(64), 0, stream, fb_count, plain_total);
 }
 
 template <class T>
