@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir
   St<T> ul, ur;
   bool need = false;
   if (valid) {
-    need = (cell_flags[k] | cell_flags[idx(g, i - di, j - dj)]) != 0;
+    need = face_flagged(g, dir, cell_flags, i, j);
     if (!need) {
       // Cell q (0..5) of the stencil is tile cell (tx + 3 - 3 + q) along the
       // normal: cells i-3 .. i+2.
@@ -540,14 +540,7 @@ __global__ __launch_bounds__(256) void cheap_reconstruct_kernel(GridD g, int dir
   // a flagged face at most 3 faces away along the face line (the plain flux).
   // With no flagged face that close it stores only its L == R flag; the two
   // state planes keep stale values nothing reads.
-  bool read_back = false;
-#pragma unroll
-  for (int d = -3; d <= 3; ++d) {
-    const int ii = i + (dir == 1 ? d : 0), jj = j + (dir == 0 ? d : 0);
-    if (ii < -g.ng || jj < -g.ng || ii >= g.nx + g.ng || jj >= g.ny + g.ng) continue;
-    read_back = read_back || (cell_flags[idx(g, ii, jj)] | cell_flags[idx(g, ii - di, jj - dj)]) != 0;
-  }
-  if (read_back) {
+  if (face_states_read_back(g, dir, cell_flags, i, j)) {
     write_faces(out, eq_flag, k, ul, ur);
   } else {
     bool eq = true;
@@ -783,7 +776,7 @@ __global__ __launch_bounds__(256) void flux_plain_list_kernel(GridD g, int dir, 
   const int j = static_cast<int>(k / static_cast<std::size_t>(g.pitch)) - g.ng;
   if (i < 0 || j < 0 || i > g.nx || j > g.ny) return;
   if (periodic_dir && (dir == 0 ? i == g.nx : j == g.ny)) return;
-  if ((cell_flags[k] | cell_flags[idx(g, i - (dir == 0), j - (dir == 1))]) != 0) {
+  if (face_flagged(g, dir, cell_flags, i, j)) {
     flux_face<T, ORDER, false>(g, dir, periodic_dir, gamma, in, flux, rate, i, j);
   } else {
     St<T> baseL, baseR;

@@ -249,6 +249,28 @@ __host__ __device__ __noinline__ inline double one_sided_average(const double* s
   return cr_sum_loop(terms, n);
 }
 
+// Learned form: a face normal to `dir` (x faces dir 0, y faces dir 1) is
+// flagged when either of its two cells is.
+__host__ __device__ __forceinline__ bool face_flagged(const GridD& g, int dir, const uint8_t* cell_flags, int i, int j) {
+  return (cell_flags[idx(g, i, j)] | cell_flags[idx(g, i - (dir == 0), j - (dir == 1))]) != 0;
+}
+
+// Whether an unflagged face's interface states are read back: a flagged face
+// at most 3 faces away along the face line (the transverse Gauss recovery's
+// stencil) reads them.  Faces of the line outside the padded grid are not
+// faces of any launch.
+__host__ __device__ __forceinline__ bool face_states_read_back(const GridD& g, int dir, const uint8_t* cell_flags,
+                                                               int i, int j) {
+  bool read_back = false;
+#pragma unroll
+  for (int d = -3; d <= 3; ++d) {
+    const int ii = i + (dir == 1 ? d : 0), jj = j + (dir == 0 ? d : 0);
+    if (ii < -g.ng || jj < -g.ng || ii >= g.nx + g.ng || jj >= g.ny + g.ng) continue;
+    read_back = read_back || face_flagged(g, dir, cell_flags, ii, jj);
+  }
+  return read_back;
+}
+
 // Centred high-order cell average (8/6/4/2 taps) when the centred stencil
 // lies inside the padded grid (every interior cell); false in the outer ghost
 // layers, whose values need the one-sided closure.

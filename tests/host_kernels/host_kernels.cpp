@@ -201,4 +201,19 @@ void hk_corner(const HkGrid* hg, int order, const int32_t* flags4, const double*
     for (int i = 0; i <= g.nx; ++i) ez[idx(g, i, j)] = corner_ez<double>(g, order, m, fx, fy, cell_ez, eqx, eqy, i, j);
 }
 
+// Learned form, faces normal to `dir` over the cheap reconstruction's range
+// (dir 0: i in [0,nx], j in [-ng,ny+ng); dir 1 transposed): flagged[k] and
+// read_back[k] at the face's padded index k.
+void hk_face_read_back(const HkGrid* hg, int dir, const uint8_t* cell_flags, uint8_t* flagged, uint8_t* read_back) {
+  const GridD g = grid_of(hg);
+  const int ilo = dir == 0 ? 0 : -g.ng, ihi = dir == 0 ? g.nx : g.nx + g.ng - 1;
+  const int jlo = dir == 0 ? -g.ng : 0, jhi = dir == 0 ? g.ny + g.ng - 1 : g.ny;
+  for (int j = jlo; j <= jhi; ++j)
+    for (int i = ilo; i <= ihi; ++i) {
+      const std::size_t k = idx(g, i, j);
+      flagged[k] = face_flagged(g, dir, cell_flags, i, j) ? 1 : 0;
+      read_back[k] = face_states_read_back(g, dir, cell_flags, i, j) ? 1 : 0;
+    }
+}
+
 }  // extern "C"

@@ -37,6 +37,7 @@ class HostKernels:
                                         C.c_void_p, C.c_void_p]
         L.hk_corner.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hk_face_read_back.argtypes = [C.POINTER(OrcGrid), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         self.lib = L
 
     def collocate(self, g, axis, co, face):
@@ -86,3 +87,12 @@ class HostKernels:
         self.lib.hk_cell_ez_tensor(C.byref(g), order, gamma, _planes(u8), bxc.ctypes.data, byc.ctypes.data,
                                    out.ctypes.data)
         return out
+
+    def face_read_back(self, g, d, cell_flags):
+        """(flagged, read_back) of the learned form's faces normal to `d`, as
+        uint8 planes over the padded grid."""
+        flags = np.ascontiguousarray(cell_flags, dtype=np.uint8)
+        flagged = np.zeros_like(flags)
+        read_back = np.zeros_like(flags)
+        self.lib.hk_face_read_back(C.byref(g), d, flags.ctypes.data, flagged.ctypes.data, read_back.ctypes.data)
+        return flagged, read_back

@@ -253,3 +253,51 @@ def test_cell_ez_tensor_recovery_matches_oracle(oracle, hk, kind, order):
     cmask[ng:ng + ny + 1, ng:ng + nx + 1] = True
     cmask = cmask.ravel()
     assert np.abs(ez - ref["ez"])[cmask].max() <= 1e-14 * np.abs(ref["ez"][cmask]).max()
+
+
+@pytest.mark.parametrize("d", [0, 1])
+@pytest.mark.parametrize("density", [0.002, 0.02, 0.2])
+def test_cheap_faces_store_every_state_a_flagged_face_reads(oracle, hk, d, density):
+    """The learned form's cheap reconstruction stores a face's interface
+    states only when face_states_read_back says so.  Brute force over the read
+    pattern of the plain flux (flux_face: the 7 faces at offsets -3..3 along
+    the face line, the same normal index): every face a flagged face in
+    [0,nx] x [0,ny] reads must be marked, on random flag fields, edges and
+    periodic high faces included.  Faces no flagged face reads are left
+    unmarked (the predicate is the exact 7-face window, not a blanket)."""
+    nx, ny, ng = 23, 19, 4
+    g = oracle.grid(nx, ny, ng)
+    rng = np.random.default_rng(int(density * 1000) + d)
+    px, py = nx + 2 * ng, ny + 2 * ng
+    flags = (rng.random(px * py) < density).astype(np.uint8)
+    flagged, read_back = hk.face_read_back(g, d, flags)
+
+    def k(i, j):
+        return (i + ng) + px * (j + ng)
+
+    if d == 0:
+        faces = [(i, j) for j in range(-ng, ny + ng) for i in range(0, nx + 1)]
+    else:
+        faces = [(i, j) for j in range(0, ny + 1) for i in range(-ng, nx + ng)]
+    # Flagged test restated: either adjacent cell flagged.
+    for i, j in faces:
+        a, b = (i - 1, j) if d == 0 else (i, j - 1)
+        assert flagged[k(i, j)] == (flags[k(i, j)] | flags[k(a, b)] != 0)
+    needed = set()
+    for i, j in faces:
+        if not (0 <= i <= nx and 0 <= j <= ny) or not flagged[k(i, j)]:
+            continue
+        for t in range(-3, 4):
+            needed.add((i, j + t) if d == 0 else (i + t, j))
+    face_set = set(faces)
+    assert needed <= face_set
+    for f in needed:
+        assert read_back[k(*f)] == 1, f
+    # Exactly the window: a face is marked iff a flagged face of the padded
+    # range lies within 3 along its line.
+    for i, j in faces:
+        near = any(flagged[k(*f)] for f in ((i, j + t) if d == 0 else (i + t, j) for t in range(-3, 4))
+                   if f in face_set)
+        assert read_back[k(i, j)] == (1 if near else 0), (i, j)
+    if density == 0.002:
+        assert read_back.sum() < len(faces) // 2
